@@ -1,0 +1,145 @@
+/*
+ * jst_gpu.h — C ABI of the MI355X (gfx950) JST + Spalart-Allmaras solver library
+ * (libjstgpu.so).  Plain pointers and sizes only; no torch or HIP types cross it.
+ *
+ * The reference (jst-rs) has no FFI: its seam is the crate's Rust API.  Each entry point
+ * below names the reference item it replaces.  Handles are opaque and not thread-safe:
+ * drive one context from one host thread; distinct contexts may run concurrently (each
+ * owns its own HIP stream).  The caller owns every host buffer; nothing is retained after
+ * a call returns.
+ */
+#ifndef JST_GPU_H
+#define JST_GPU_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  1..6 map 1:1 onto jst-rs MeshError (mesh.rs:18-25); CONFIG is the
+ * InvalidData io::Error of Config::from_str (config.rs:243-248); NONPHYSICAL is
+ * NonPhysical (state.rs:241-259, solver.rs:151-185). */
+typedef enum {
+    JST_OK = 0,
+    JST_ERR_IO = 1,
+    JST_ERR_BAD_HEADER = 2,
+    JST_ERR_EMPTY = 3,
+    JST_ERR_COUNT_MISMATCH = 4,
+    JST_ERR_BAD_POINT = 5,
+    JST_ERR_TOO_SMALL = 6,
+    JST_ERR_NONPHYSICAL = 7,
+    JST_ERR_CONFIG = 8,
+    JST_ERR_HIP = 9,
+    JST_ERR_ARG = 10
+} jst_status;
+
+typedef enum {
+    JST_FP64 = 0,  /* plain path: every operation in FP64, reference operation order */
+    JST_MIXED = 1  /* FP32 stencil arithmetic, FP64 geometry/accumulation/reductions */
+} jst_precision;
+
+typedef struct jst_ctx jst_ctx;
+
+/* jst-rs state.rs:241-259 NonPhysical {i, j, rho, p} of the first row-major offender. */
+typedef struct {
+    long i, j;
+    double rho, p;
+} jst_nonphysical;
+
+/* jst-rs solver.rs:38-43 RunReport. */
+typedef struct {
+    long steps;
+    double residual;
+    double totaltime;
+    int converged;
+} jst_report;
+
+/* Per-step callback of Solver::run (solver.rs:203-226, `on_step(step, residual)`). */
+typedef void (*jst_step_cb)(long step, double residual, void *user);
+
+/* Thread-local text of the last error, worded like the reference's Display impls
+ * (mesh.rs:27-44, state.rs:249-257).  Never NULL. */
+const char *jst_last_error(void);
+
+/* Config::from_str (config.rs:243-248) + Config::finish (251-284) on the host only.
+ * derived_out (may be NULL) receives 20 doubles: cv cp cw1 cv1_cubed c_inf u_inf v_inf
+ * rho_inf mu_inf nut_inf supersonic suth_num inv_t_ref inv_pr inv_prt inv_kappa2 cw3_6
+ * one_plus_cw3_6 mu0_eff halo.  No GPU is touched. */
+jst_status jst_config_check(const char *config_json, double *derived_out);
+
+/* Mesh::parse (mesh.rs:109-166) incl. trim_closing_duplicate (169-189) on the host only.
+ * Fills n_rings / n_theta; xy (may be NULL) receives 2*n_rings*n_theta doubles. */
+jst_status jst_mesh_parse(const char *mesh_text, int *n_rings, int *n_theta, double *xy,
+                          size_t xy_len);
+
+/* Solver::new(cfg, &Mesh) (solver.rs:47-62): geometry (FP64, on the GPU), freestream
+ * initialisation and boundary state.  Validates exactly like Config::from_str and
+ * Mesh::parse and returns their status. */
+jst_status jst_create(const char *config_json, const char *mesh_text, jst_precision prec,
+                      jst_ctx **out);
+/* Solver::from_paths (solver.rs:289-293). */
+jst_status jst_create_from_paths(const char *config_path, const char *mesh_path,
+                                 jst_precision prec, jst_ctx **out);
+void jst_destroy(jst_ctx *ctx);
+
+/* Solver::{ni, nj} (solver.rs:295-306). */
+jst_status jst_dims(const jst_ctx *ctx, int *ni, int *nj);
+
+/* nsteps x Solver::advance (solver.rs:97-120) followed by Solver::residual (190-200):
+ * res_out (may be NULL) receives one residual per completed step.  On a non-physical state
+ * returns JST_ERR_NONPHYSICAL and fills err (may be NULL). */
+jst_status jst_advance(jst_ctx *ctx, int nsteps, double *res_out, jst_nonphysical *err);
+
+/* Solver::run (solver.rs:203-226).  max_steps <= 0 uses solver.iteration.  cb (may be
+ * NULL) fires once per step in order; residuals are read back from the device in batches,
+ * the step count is exact (the device stops on convergence). */
+jst_status jst_run(jst_ctx *ctx, long max_steps, jst_step_cb cb, void *user, jst_report *out,
+                   jst_nonphysical *err);
+
+/* Solver::residual (solver.rs:190-200) of the last completed step (NaN before any). */
+jst_status jst_residual(const jst_ctx *ctx, double *out);
+
+/* timestep::compute (timestep.rs:24-47) + Solver::residual_terms (solver.rs:86-94): one
+ * stage's Fc, Fv, Fd, S, gradients and SA auxiliaries on the current state, without
+ * touching U.  dt_out receives the global minimum dt.  Read the terms with jst_get_field. */
+jst_status jst_residual_terms(jst_ctx *ctx, double *dt_out);
+
+/* Field<T>::to_interior_vec (field.rs:343-345): interior values row-major NI x NJ
+ * ((NI+1) x NJ for tau_* faces).  Names:
+ *   state:     rho p T u v E H c nut U0..U4 localdt
+ *   terms:     Fc0..4 Fv0..4 Fd0..4 S grad0..7 (dudx dudy dvdx dvdy dTdx dTdy dnudx dnudy)
+ *              chi fv1  (valid after jst_residual_terms)
+ *   geometry:  vol cx cy wall_dist inv_vol inv_wd2 tau_nx tau_ny tau_mx tau_my n_nx n_ny
+ *              n_mx n_my
+ * n is the capacity of dst in doubles. */
+jst_status jst_get_field(jst_ctx *ctx, const char *name, double *dst, size_t n);
+
+/* Overwrite the interior conserved state (5 planes of NI x NJ, row-major) — test
+ * injection such as rho = -1 (solver.rs:358-365).  Runs the unpack check and returns
+ * JST_ERR_NONPHYSICAL with err filled when the state is not physical. */
+jst_status jst_set_state(jst_ctx *ctx, const double *u_planes, jst_nonphysical *err);
+
+/* No counterpart in the reference (it computes no forces): wall force from the wall
+ * tau-face momentum fluxes, F = -sum_j (Fc - Fv)_tau(0,j)[mx,my], and CL/CD by the angle
+ * of attack over q_inf * L.  Summed in FP64 in j order. */
+jst_status jst_forces(jst_ctx *ctx, double *fx, double *fy, double *cl, double *cd);
+
+/* Solver::write_csv (solver.rs:229-255): columns i,j,x,y,rho,p,T,u,v,miubl, 1-based,
+ * {:.8e}. */
+jst_status jst_write_csv(jst_ctx *ctx, const char *path);
+
+/* Step counter and accumulated physical time (Solver::{step, totaltime}). */
+jst_status jst_progress(const jst_ctx *ctx, long *step, double *totaltime);
+
+/* Timing hook for benchmarks (no reference counterpart): runs nsteps steps bracketed by HIP
+ * events on the context's stream.  total_ms = whole region; when stage_ms is not NULL a
+ * second pass of nsteps steps records events around every stage-kernel launch and
+ * stage_ms receives the mean stage-kernel duration.  The device does not stop on
+ * convergence during timing. */
+jst_status jst_time_steps(jst_ctx *ctx, int nsteps, double *total_ms, double *stage_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,73 @@
+// jst_dev.h — structures shared by the host driver and the gfx950 kernels.
+//
+// One "case" is one O-mesh solve.  Its conserved state lives in HBM as structure-of-arrays
+// planes [5][NI][pitch] with j (the circumferential index) fastest, so a wavefront's 64
+// lanes read 64 consecutive cells of one ring (coalesced along the reference's fast axis,
+// jst-rs field.rs:139-141).  No halo is stored: ghost cells (wall mirror, far-field Riemann
+// state, periodic seam) are synthesised by the kernels' tile loaders from interior data.
+#pragma once
+
+#include <stdint.h>
+
+#define JST_H 3                   // ghost layers (config.json IM; jst-rs config.rs:33-35)
+#define JST_ERR_NONE 0xFFFFFFFFFFFFFFFFull
+#define JST_INF_BITS 0x7FF0000000000000ull
+
+// Per-case run control, resident in device memory and advanced by the kernels themselves,
+// so that many steps can be queued (or replayed from a HIP graph) without host round trips.
+struct CaseCtl {
+    double dt;                    // frozen global dt of the step in progress (solver.rs:109-110)
+    unsigned long long dt_acc;    // atomicMin accumulator of the next dt (bits of a positive double)
+    double totaltime;             // accumulated physical time (solver.rs:111)
+    long long step;               // completed steps
+    long long stop_step;          // the device stops advancing when step reaches this
+    double res_first;             // residual of step 1 (relative-drop stop criterion)
+    double residual;              // residual of the last completed step
+    unsigned long long err_key;   // (stage sequence << 32) | row-major cell index of first offender
+    int status;                   // JST_RUN_*
+    int rot;                      // buffer rotation 0..2 (U^n lives in ubuf[rot])
+    unsigned int blocks_done;     // last-block-done counter of the step-end kernel
+    int timing;                   // 1: ignore convergence / stop_step (benchmark timing)
+};
+
+enum { JST_RUN_ACTIVE = 0, JST_RUN_CONVERGED = 1, JST_RUN_STOPPED = 2, JST_RUN_NONPHYSICAL = 3 };
+
+// Physical and scheme constants of a case (jst-rs config.rs Derived + inputs).
+struct PhysConst {
+    double gamma, r_gas, cfl, k2, k4;
+    double t_inf, p_inf, rho_inf, u_inf, v_inf, c_inf, nut_inf;
+    double suth_num, inv_t_ref, t_suth, inv_pr, inv_prt, cp, sigma, cv1_cubed;
+    double cb1, cb2, ct3, ct4, cw1, cw2, cw3_6, one_plus_cw3_6, fv3, inv_kappa2, rmax;
+    double targetres, residual_drop;
+    int supersonic, lts;
+};
+
+// Device view of one case.  FP64 plain path: every array is double.
+struct CaseDev {
+    int ni, nj, pitch;            // pitch >= nj, multiple of 16 elements
+    int ntiles;                   // tiles of this case in the tile list
+    double *ubuf[3];              // 3 rotating buffers of 5 planes each
+    const double *tnx, *tny;      // tau-face normals, (ni+1) x pitch
+    const double *tmx, *tmy;      // tau-face midpoints, (ni+1) x pitch
+    const double *nnx, *nny;      // n-face normals, ni x pitch
+    const double *vol, *inv_vol, *inv_wd2, *cx, *cy;
+    double *ratio;                // frozen V/dt_local per cell (jst-rs dissipation.rs:44-52)
+    double *localdt;              // frozen dt_local per cell (timestep.rs:24-47)
+    double *partial;              // per-tile partial sums of the step-end reduction
+    double *res_hist;             // residual ring buffer, res_cap entries
+    int res_cap;
+    CaseCtl *ctl;
+    double *dbg;                  // residual-terms export planes (may be null)
+    PhysConst k;
+};
+
+// Tile descriptor: x = case, y = i0, z = j0, w = tile index within the case.
+struct TileDesc {
+    int c, i0, j0, t;
+};
+
+// Export planes written by the residual-terms mode of the stage kernel.
+enum {
+    DBG_FC = 0, DBG_FV = 5, DBG_FD = 10, DBG_S = 15, DBG_GRAD = 16, DBG_CHI = 24, DBG_FV1 = 25,
+    DBG_NPLANES = 26
+};

@@ -1,0 +1,891 @@
+// jst_fp64.hip — FP64 plain path for gfx950.
+//
+// Built with -ffp-contract=off: the reference (Rust) never fuses a*b+c, and this path is
+// the parity path (per-residual relative error <= 1e-10 against the reference algorithm).
+//
+// Kernels
+//   geom_*          FP64 metric precompute (jst-rs geometry.rs:76-216), once per case.
+//   stage_kernel    one fused RK stage on a TI x TJ tile: ghost synthesis (boundary.rs),
+//                   central flux (convection.rs), Green-Gauss gradient (gradient.rs),
+//                   viscous/SA diffusion (viscous.rs), JST dissipation (dissipation.rs),
+//                   SA source (source.rs), update + non-physical check (solver.rs:123-185).
+//   post_kernel     end of step: L2 density residual (solver.rs:190-200) with a fixed-order
+//                   reduction, next step's local/global dt (timestep.rs:24-57), step control.
+//   forces_kernel   wall force / CL / CD (no reference counterpart).
+#include <hip/hip_runtime.h>
+
+#include "jst_dev.h"
+#include "jst_launch.h"
+
+namespace {
+
+__device__ __forceinline__ int wrapj(int j, int nj)
+{
+    j %= nj;
+    return j < 0 ? j + nj : j;
+}
+
+__device__ __forceinline__ size_t cidx(int i, int j, int pitch) { return (size_t)i * pitch + j; }
+
+struct Prim {
+    double rho, vx, vy, p, t, e, nut;
+};
+
+// jst-rs solver.rs:151-185 (Solver::unpack), without the error branch.
+__device__ __forceinline__ void unpack(const PhysConst &k, const double u[5], Prim &q)
+{
+    double rho = u[0];
+    double inv = 1.0 / rho;
+    q.rho = rho;
+    q.vx = u[1] * inv;
+    q.vy = u[2] * inv;
+    q.e = u[3] * inv;
+    q.p = (k.gamma - 1.0) * (u[3] - rho * (q.vx * q.vx + q.vy * q.vy) * 0.5);
+    q.t = q.p / (k.r_gas * rho);
+    q.nut = u[4] * inv;
+}
+
+// jst-rs state.rs:142-156 (Cells::pack)
+__device__ __forceinline__ void pack(const Prim &q, double u[5])
+{
+    u[0] = q.rho;
+    u[1] = q.rho * q.vx;
+    u[2] = q.rho * q.vy;
+    u[3] = q.rho * q.e;
+    u[4] = q.rho * q.nut;
+}
+
+__device__ __forceinline__ void load_u(const double *U, size_t plane, size_t c, double u[5])
+{
+#pragma unroll
+    for (int m = 0; m < 5; ++m) u[m] = U[m * plane + c];
+}
+
+// jst-rs boundary.rs:112-185 (riemann_face + extrapolate_nut) for far-field column jw,
+// from the current interior state U.  Out: {rho, p, t, vx, vy, e, nut}.
+__device__ void far_state(const CaseDev &C, const double *U, int jw, Prim &s)
+{
+    const PhysConst &k = C.k;
+    const int ni = C.ni, pitch = C.pitch;
+    const size_t plane = (size_t)ni * pitch;
+    Prim q[3];
+    for (int m = 0; m < 3; ++m) {
+        double u[5];
+        load_u(U, plane, cidx(ni - 1 - m, jw, pitch), u);
+        unpack(k, u, q[m]);
+    }
+    const size_t f = cidx(ni, jw, pitch);
+    const double nx = C.tnx[f], ny = C.tny[f], fmx = C.tmx[f], fmy = C.tmy[f];
+    const double len = hypot(nx, ny);
+    const double gamma = k.gamma;
+    const double vn_inf = (k.u_inf * nx + k.v_inf * ny) / len;
+    const double vt_inf = (k.u_inf * ny - k.v_inf * nx) / len;
+    const double vn_in = (q[0].vx * nx + q[0].vy * ny) / len;
+    const double vt_in = (q[0].vx * ny - q[0].vy * nx) / len;
+    const double c_in = sqrt(k.r_gas * gamma * q[0].t);
+    double c_face, vn_face, vt_face;
+    bool inflow;
+    if (!k.supersonic) {
+        double r_in = vn_in + 2.0 * c_in / (gamma - 1.0);
+        double r_inf = vn_inf - 2.0 * k.c_inf / (gamma - 1.0);
+        double vn = 0.5 * (r_in + r_inf);
+        double c = (gamma - 1.0) / 4.0 * (r_in - r_inf);
+        inflow = vn <= 0.0;
+        c_face = c;
+        vn_face = vn;
+        vt_face = inflow ? vt_inf : vt_in;
+    } else if (vn_inf <= 0.0) {
+        c_face = k.c_inf;
+        vn_face = vn_inf;
+        vt_face = vt_inf;
+        inflow = true;
+    } else {
+        c_face = c_in;
+        vn_face = vn_in;
+        vt_face = vt_in;
+        inflow = false;
+    }
+    const double t = c_face * c_face / (k.r_gas * gamma);
+    const double rho = k.rho_inf * pow(t / k.t_inf, 1.0 / (gamma - 1.0));
+    const double p = rho * k.r_gas * t;
+    const double vx = (vt_face * ny + vn_face * nx) / len;
+    const double vy = (-vt_face * nx + vn_face * ny) / len;
+    const double e = p / (rho * (gamma - 1.0)) + 0.5 * (vx * vx + vy * vy);
+    double nut;
+    if (inflow) {
+        nut = k.nut_inf;
+    } else {
+        const double FLOOR = 1e-10;
+        double dist[3];
+        for (int m = 0; m < 3; ++m) {
+            size_t c = cidx(ni - 1 - m, jw, pitch);
+            dist[m] = hypot(C.cx[c] - fmx, C.cy[c] - fmy);
+        }
+        double l1 = dist[0], l2 = dist[1], l3 = dist[2];
+        double d12 = l1 - l2, d13 = l1 - l3, d23 = l2 - l3;
+        if (fabs(d12) < 1e-14 || fabs(d13) < 1e-14 || fabs(d23) < 1e-14) {
+            nut = fmax(q[0].nut, FLOOR);
+        } else {
+            double v = l2 * l3 / (d12 * d13) * q[0].nut + l1 * l3 / (-d12 * d23) * q[1].nut +
+                       l1 * l2 / (d23 * -d13) * q[2].nut;
+            nut = v > FLOOR ? v : FLOOR;
+        }
+    }
+    s.rho = rho;
+    s.p = p;
+    s.t = t;
+    s.vx = vx;
+    s.vy = vy;
+    s.e = e;
+    s.nut = nut;
+}
+
+// jst-rs convection.rs:25-40 (euler_flux) on the face state fu = (U_L + U_R)/2.
+__device__ __forceinline__ void euler_flux(const double ul[5], const double ur[5], double nx,
+                                           double ny, double gamma, double out[5])
+{
+    double fu[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) fu[m] = 0.5 * (ul[m] + ur[m]);
+    double rho = fu[0];
+    double inv_rho = 1.0 / rho;
+    double u = fu[1] * inv_rho;
+    double v = fu[2] * inv_rho;
+    double rho_e = fu[3];
+    double p = (gamma - 1.0) * (rho_e - rho * (u * u + v * v) * 0.5);
+    double vn = nx * u + ny * v;
+    out[0] = rho * vn;
+    out[1] = fu[1] * vn + p * nx;
+    out[2] = fu[2] * vn + p * ny;
+    out[3] = (rho_e + p) * vn;
+    out[4] = fu[4] * vn;
+}
+
+// jst-rs dissipation.rs:29-31
+__device__ __forceinline__ double pressure_sensor(double pm, double p0, double pp)
+{
+    return fabs((pm - 2.0 * p0 + pp) / (pm + 2.0 * p0 + pp));
+}
+
+// jst-rs viscous.rs:37-71 (diffusion_tensor).  d[0..3] = D_x[1..4], d[4..7] = D_y[1..4]
+// (component 0 of both columns is identically zero).
+__device__ __forceinline__ void diffusion_tensor(const PhysConst &k, double t, double rho_nu,
+                                                 double vx, double vy, const double g[8],
+                                                 double d[8], double &chi_out, double &fv1_out)
+{
+    double r = t * k.inv_t_ref;
+    double mu = r * sqrt(r) * k.suth_num / (t + k.t_suth); // config.rs:288-291
+    double chi = rho_nu / mu;
+    double c3 = chi * chi * chi;
+    double f1 = c3 / (c3 + k.cv1_cubed);
+    double mu_t = rho_nu * f1;
+    double mu_eff = mu + mu_t;
+    double lam_eff = mu * k.inv_pr + mu_t * k.inv_prt;
+    double txx = mu_eff * (4.0 / 3.0 * g[0] - 2.0 / 3.0 * g[3]);
+    double tyy = mu_eff * (4.0 / 3.0 * g[3] - 2.0 / 3.0 * g[0]);
+    double txy = mu_eff * (g[1] + g[2]);
+    double qx = lam_eff * k.cp * g[4];
+    double qy = lam_eff * k.cp * g[5];
+    double nu_diff = k.sigma * (mu + rho_nu);
+    d[0] = txx;
+    d[1] = txy;
+    d[2] = vx * txx + vy * txy + qx;
+    d[3] = nu_diff * g[6];
+    d[4] = txy;
+    d[5] = tyy;
+    d[6] = vx * txy + vy * tyy + qy;
+    d[7] = nu_diff * g[7];
+    chi_out = chi;
+    fv1_out = f1;
+}
+
+// LLVM's constant powi expansion order (x^2, x^4, x^2*x^4), as Rust's f64::powi(6).
+__device__ __forceinline__ double pow6(double x)
+{
+    double x2 = x * x;
+    double x4 = x2 * x2;
+    return x2 * x4;
+}
+
+// jst-rs source.rs:45-88 for one cell (already multiplied by the cell volume).
+__device__ __forceinline__ double sa_source(const PhysConst &k, double rho, double rho_nu,
+                                            double chi, double fv1, const double g[8],
+                                            double inv_dist2, double vol)
+{
+    double cb1_over_kappa2 = k.cb1 * k.inv_kappa2;
+    double cb2_sigma = k.cb2 * k.sigma;
+    double nu_tilde = rho_nu / rho;
+    double inv_kd2 = inv_dist2 * k.inv_kappa2;
+    double ft2 = k.ct3 * exp(-k.ct4 * chi * chi);
+    double fv2 = 1.0 - chi / (1.0 + chi * fv1);
+    double s = k.fv3 * fabs(g[2] - g[1]);
+    double s_tilde = fmax(s + nu_tilde * inv_kd2 * fv2, 1e-10);
+    double prod = k.cb1 * (1.0 - ft2) * s_tilde * rho_nu;
+    double r = fmin(nu_tilde / s_tilde * inv_kd2, k.rmax);
+    double gg = r + k.cw2 * (pow6(r) - r);
+    double fw = gg * cbrt(sqrt(k.one_plus_cw3_6 / (pow6(gg) + k.cw3_6)));
+    double destr = (k.cw1 * fw - cb1_over_kappa2 * ft2) * rho * (nu_tilde * nu_tilde * inv_dist2);
+    double diff = cb2_sigma * rho * (g[6] * g[6] + g[7] * g[7]);
+    return (prod - destr + diff) * vol;
+}
+
+} // namespace
+
+// ---------------------------------------------------------------------------------------
+// geometry (jst-rs geometry.rs:76-216): nodes are (ring, theta) ring-major, theta wraps.
+// ---------------------------------------------------------------------------------------
+
+__global__ void geom_cells_kernel(const double *xy, int n_theta, int ni, int nj, int pitch,
+                                  double *vol, double *cx, double *cy)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nj || i >= ni) return;
+    auto node = [&](int r, int t, double &x, double &y) {
+        size_t q = (size_t)r * n_theta + (t % n_theta);
+        x = xy[2 * q];
+        y = xy[2 * q + 1];
+    };
+    double ax, ay, bx, by, qx, qy, dx, dy;
+    node(i, j, ax, ay);
+    node(i + 1, j, bx, by);
+    node(i + 1, j + 1, qx, qy);
+    node(i, j + 1, dx, dy);
+    double d1x = qx - ax, d1y = qy - ay;
+    double d2x = bx - dx, d2y = by - dy;
+    double v = 0.5 * fabs(d1x * d2y - d1y * d2x);
+    double cr0 = ax * by - bx * ay;
+    double cr1 = bx * qy - qx * by;
+    double cr2 = qx * dy - dx * qy;
+    double cr3 = dx * ay - ax * dy;
+    double sgn = 0.5 * (cr0 + cr1 + cr2 + cr3);
+    double sx = (ax + bx) * cr0 + (bx + qx) * cr1 + (qx + dx) * cr2 + (dx + ax) * cr3;
+    double sy = (ay + by) * cr0 + (by + qy) * cr1 + (qy + dy) * cr2 + (dy + ay) * cr3;
+    size_t c = cidx(i, j, pitch);
+    vol[c] = v;
+    cx[c] = fabs(sgn) > 1e-30 ? sx / (6.0 * sgn) : 0.0;
+    cy[c] = fabs(sgn) > 1e-30 ? sy / (6.0 * sgn) : 0.0;
+}
+
+__global__ void geom_faces_kernel(const double *xy, int n_theta, int ni, int nj, int pitch,
+                                  double *tnx, double *tny, double *tmx, double *tmy,
+                                  double *nnx, double *nny, double *nmx, double *nmy)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nj || i > ni) return;
+    auto node = [&](int r, int t, double &x, double &y) {
+        size_t q = (size_t)r * n_theta + (t % n_theta);
+        x = xy[2 * q];
+        y = xy[2 * q + 1];
+    };
+    size_t c = cidx(i, j, pitch);
+    double ax, ay, bx, by;
+    node(i, j, ax, ay);
+    node(i, j + 1, bx, by);
+    double dx = bx - ax, dy = by - ay;
+    tnx[c] = dy; // tau normal (dy, -dx) points radially outward
+    tny[c] = -dx;
+    tmx[c] = 0.5 * (ax + bx);
+    tmy[c] = 0.5 * (ay + by);
+    if (i < ni) {
+        node(i + 1, j, bx, by);
+        dx = bx - ax;
+        dy = by - ay;
+        nnx[c] = -dy; // n normal (-dy, dx) points toward +j
+        nny[c] = dx;
+        nmx[c] = 0.5 * (ax + bx);
+        nmy[c] = 0.5 * (ay + by);
+    }
+}
+
+// geometry.rs:192-216: window = min(max(15, nj/5), nj/2), then 1/V and 1/d^2 (95-101).
+__global__ void geom_walldist_kernel(int ni, int nj, int pitch, const double *cx, const double *cy,
+                                     const double *tmx, const double *tmy, const double *vol,
+                                     double *wd, double *inv_vol, double *inv_wd2)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.y;
+    if (j >= nj || i >= ni) return;
+    int w = nj / 5 > 15 ? nj / 5 : 15;
+    if (w > nj / 2) w = nj / 2;
+    size_t c = cidx(i, j, pitch);
+    double px = cx[c], py = cy[c];
+    double best = __builtin_huge_val();
+    for (int dk = -w; dk <= w; ++dk) {
+        int k = wrapj(j + dk, nj);
+        double ex = px - tmx[k], ey = py - tmy[k];
+        double d = sqrt(ex * ex + ey * ey);
+        if (d < best) best = d;
+    }
+    wd[c] = best;
+    inv_vol[c] = 1.0 / vol[c];
+    inv_wd2[c] = 1.0 / (best * best);
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused RK stage.
+//
+// A block owns a TI x TJ tile of interior cells.  LDS holds the tile plus a 3-cell ring
+// ("R3": U, p and the gradient primitives), the tile plus a 1-cell ring ("R1": viscous
+// tensor D and the frozen V/dt), and the pressure sensors.  Ghost cells of the ring are
+// synthesised on load: rows < 0 mirror rows -1-k (boundary.rs:30-51), rows >= NI take the
+// far-field Riemann state of their column (boundary.rs:87-155), columns wrap modulo NJ
+// (boundary.rs:54-72).  Every face flux is evaluated from identical inputs by both cells
+// that share it, so the discrete scheme stays exactly conservative.
+// ---------------------------------------------------------------------------------------
+
+template <int TI, int TJ>
+struct StageSmem {
+    static constexpr int H3 = TI + 6, W3 = TJ + 6; // R3 rectangle
+    static constexpr int H1 = TI + 2, W1 = TJ + 2; // R1 rectangle
+    double u[5][H3][W3];
+    double p[H3][W3];
+    double t[H3][W3];
+    double vx[H3][W3];
+    double vy[H3][W3];
+    double nu[H3][W3];
+    double si[TI + 4][TJ]; // sensor_i rows i0-2 .. i0+TI+1
+    double sj[TI][TJ + 4]; // sensor_j cols j0-2 .. j0+TJ+1
+    double ratio[H1][W1];  // V/dt rows clamp(i0-1 .. i0+TI), cols j0-1 .. j0+TJ (wrapped)
+    double d[8][H1][W1];   // viscous tensor on R1
+    double src[TI][TJ];
+};
+
+template <int TI, int TJ, int NT, bool TERMS>
+__global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ cases,
+                                                   const TileDesc *__restrict__ tiles, int stage,
+                                                   double alpha)
+{
+    using S = StageSmem<TI, TJ>;
+    constexpr int H3 = S::H3, W3 = S::W3, H1 = S::H1, W1 = S::W1;
+    __shared__ S sm;
+
+    const TileDesc tl = tiles[blockIdx.x];
+    const CaseDev &C = cases[tl.c];
+    const PhysConst &k = C.k;
+    CaseCtl *ctl = C.ctl;
+    const int ni = C.ni, nj = C.nj, pitch = C.pitch;
+    const size_t plane = (size_t)ni * pitch;
+    const int i0 = tl.i0, j0 = tl.j0;
+    const int tid = threadIdx.x;
+
+    const int status = *(volatile int *)&ctl->status;
+    const long long step = *(volatile long long *)&ctl->step;
+    const unsigned long long ek = *(volatile unsigned long long *)&ctl->err_key;
+    const int rot = *(volatile int *)&ctl->rot;
+    const long long seq = step * 5 + stage;
+    if (!TERMS) {
+        if (status != JST_RUN_ACTIVE) return;
+        if (ek != JST_ERR_NONE && (long long)(ek >> 32) < seq) return;
+    }
+    // buffer rotation: stage 0 reads U^n = ubuf[rot]; stages alternate between the two others.
+    const double *src;
+    double *dst;
+    const double *former = C.ubuf[rot];
+    {
+        const int b1 = rot == 2 ? 0 : rot + 1, b2 = b1 == 2 ? 0 : b1 + 1;
+        if (TERMS) {
+            src = C.ubuf[rot];
+            dst = nullptr;
+        } else if (stage == 0) {
+            src = C.ubuf[rot];
+            dst = C.ubuf[b1];
+        } else if (stage & 1) {
+            src = C.ubuf[b1];
+            dst = C.ubuf[b2];
+        } else {
+            src = C.ubuf[b2];
+            dst = C.ubuf[b1];
+        }
+    }
+    const double dt = ctl->dt;
+
+    // ---- phase 1: R3 load with ghost synthesis ----
+    for (int idx = tid; idx < H3 * W3; idx += NT) {
+        const int r = idx / W3, c = idx - r * W3;
+        const int i = i0 - JST_H + r, jj = j0 - JST_H + c;
+        const int jw = wrapj(jj, nj);
+        double u[5];
+        Prim q;
+        if (i >= 0 && i < ni) {
+            load_u(src, plane, cidx(i, jw, pitch), u);
+            unpack(k, u, q);
+        } else if (i < 0 && i >= -JST_H) {
+            load_u(src, plane, cidx(-1 - i, jw, pitch), u);
+            unpack(k, u, q);
+            q.vx = -q.vx;
+            q.vy = -q.vy;
+            q.nut = -q.nut;
+            pack(q, u);
+        } else if (i >= ni && i < ni + JST_H) {
+            far_state(C, src, jw, q);
+            pack(q, u);
+        } else { // outside the ghost layers: never read by a valid cell
+            q.rho = 1.0; q.vx = 0.0; q.vy = 0.0; q.p = 1.0; q.t = 1.0; q.e = 1.0; q.nut = 0.0;
+            pack(q, u);
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) sm.u[m][r][c] = u[m];
+        sm.p[r][c] = q.p;
+        sm.t[r][c] = q.t;
+        sm.vx[r][c] = q.vx;
+        sm.vy[r][c] = q.vy;
+        sm.nu[r][c] = q.nut;
+    }
+    // frozen V/dt on R1: rows clamped to [0, NI-1] (dissipation.rs:59), columns wrapped.
+    for (int idx = tid; idx < H1 * W1; idx += NT) {
+        const int r = idx / W1, c = idx - r * W1;
+        int i = i0 - 1 + r;
+        i = i < 0 ? 0 : (i > ni - 1 ? ni - 1 : i);
+        sm.ratio[r][c] = C.ratio[cidx(i, wrapj(j0 - 1 + c, nj), pitch)];
+    }
+    __syncthreads();
+
+    // ---- phase 2: sensors, gradients + viscous tensor on R1, SA source on the tile ----
+    for (int idx = tid; idx < (TI + 4) * TJ; idx += NT) {
+        const int r = idx / TJ, c = idx - r * TJ; // cell (i0-2+r, j0+c) -> R3 (r+1, c+3)
+        sm.si[r][c] = pressure_sensor(sm.p[r][c + 3], sm.p[r + 1][c + 3], sm.p[r + 2][c + 3]);
+    }
+    for (int idx = tid; idx < TI * (TJ + 4); idx += NT) {
+        const int r = idx / (TJ + 4), c = idx - r * (TJ + 4); // cell (i0+r, j0-2+c)
+        sm.sj[r][c] = pressure_sensor(sm.p[r + 3][c], sm.p[r + 3][c + 1], sm.p[r + 3][c + 2]);
+    }
+    for (int idx = tid; idx < H1 * W1; idx += NT) {
+        const int r = idx / W1, c = idx - r * W1;
+        const int i = i0 - 1 + r, jj = j0 - 1 + c;
+        const int R = r + 2, Q = c + 2; // R3 coordinates of the cell
+        double g[8];
+        if (i < -1 || i > ni) continue;
+        if (i == ni) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) g[m] = 0.0; // far field (gradient.rs:97)
+        } else {
+            // gradient.rs:22-80 at (ie, jj); the wall ghost row copies row 0 (gradient.rs:93-96)
+            const int ie = i < 0 ? 0 : i;
+            const int Re = i < 0 ? R + 1 : R;
+            const int jw = wrapj(jj, nj);
+            const int jp = jw + 1 < nj ? jw + 1 : 0;
+            const double upx = C.tnx[cidx(ie + 1, jw, pitch)], upy = C.tny[cidx(ie + 1, jw, pitch)];
+            const double dnx = C.tnx[cidx(ie, jw, pitch)], dny = C.tny[cidx(ie, jw, pitch)];
+            const double lfx = C.nnx[cidx(ie, jw, pitch)], lfy = C.nny[cidx(ie, jw, pitch)];
+            const double rtx = C.nnx[cidx(ie, jp, pitch)], rty = C.nny[cidx(ie, jp, pitch)];
+            const double inv_v = C.inv_vol[cidx(ie, jw, pitch)];
+            const double(*fld[4])[W3] = {sm.vx, sm.vy, sm.t, sm.nu};
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                const double cc = fld[f][Re][Q];
+                const double f_up = 0.5 * (cc + fld[f][Re + 1][Q]);
+                const double f_dn = 0.5 * (cc + fld[f][Re - 1][Q]);
+                const double f_lf = 0.5 * (cc + fld[f][Re][Q - 1]);
+                const double f_rt = 0.5 * (cc + fld[f][Re][Q + 1]);
+                g[2 * f] = (f_up * upx - f_dn * dnx + f_rt * rtx - f_lf * lfx) * inv_v;
+                g[2 * f + 1] = (f_up * upy - f_dn * dny + f_rt * rty - f_lf * lfy) * inv_v;
+            }
+            if (i < 0) {
+                g[4] = 0.0; // adiabatic wall: no temperature gradient in the ghost
+                g[5] = 0.0;
+            }
+        }
+        double d[8], chi, fv1;
+        diffusion_tensor(k, sm.t[R][Q], sm.u[4][R][Q], sm.vx[R][Q], sm.vy[R][Q], g, d, chi, fv1);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) sm.d[m][r][c] = d[m];
+        const bool own = r >= 1 && r <= TI && c >= 1 && c <= TJ && i < ni && jj < nj;
+        if (own) {
+            const size_t cc = cidx(i, jj, pitch);
+            sm.src[r - 1][c - 1] = sa_source(k, sm.u[0][R][Q], sm.u[4][R][Q], chi, fv1, g,
+                                             C.inv_wd2[cc], C.vol[cc]);
+            if (TERMS) {
+#pragma unroll
+                for (int m = 0; m < 8; ++m) C.dbg[(DBG_GRAD + m) * plane + cc] = g[m];
+                C.dbg[DBG_CHI * plane + cc] = chi;
+                C.dbg[DBG_FV1 * plane + cc] = fv1;
+                C.dbg[DBG_S * plane + cc] = sm.src[r - 1][c - 1];
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 3: face fluxes, residual and update of the tile's own cells ----
+    const double gamma = k.gamma, half_cfl = 0.5 * k.cfl;
+    for (int o = tid; o < TI * TJ; o += NT) {
+        const int ti = o / TJ, tj = o - ti * TJ;
+        const int i = i0 + ti, j = j0 + tj;
+        if (i >= ni || j >= nj) continue;
+        const int R = ti + 3, Q = tj + 3;    // R3
+        const int R1 = ti + 1, Q1 = tj + 1;  // R1
+        const int jp = j + 1 < nj ? j + 1 : 0;
+        double fc[5], fv[5], fd[5];
+        double a[5], b[5], up[5], dn[5], rt[5], lf[5];
+        // convection (convection.rs:46-80): tau faces i (dn) and i+1 (up), n faces j (lf), j+1 (rt)
+        const size_t c_dn = cidx(i, j, pitch), c_up = cidx(i + 1, j, pitch), c_rt = cidx(i, jp, pitch);
+        const double dnx = C.tnx[c_dn], dny = C.tny[c_dn], upx = C.tnx[c_up], upy = C.tny[c_up];
+        const double lfx = C.nnx[c_dn], lfy = C.nny[c_dn], rtx = C.nnx[c_rt], rty = C.nny[c_rt];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) { a[m] = sm.u[m][R - 1][Q]; b[m] = sm.u[m][R][Q]; }
+        euler_flux(a, b, dnx, dny, gamma, dn);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a[m] = sm.u[m][R + 1][Q];
+        euler_flux(b, a, upx, upy, gamma, up);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a[m] = sm.u[m][R][Q - 1];
+        euler_flux(a, b, lfx, lfy, gamma, lf);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a[m] = sm.u[m][R][Q + 1];
+        euler_flux(b, a, rtx, rty, gamma, rt);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) fc[m] = up[m] - dn[m] + rt[m] - lf[m];
+
+        // viscous faces (viscous.rs:128-164): 0.5 (D_a + D_b) . n
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double cx0 = sm.d[m][R1][Q1], cy0 = sm.d[4 + m][R1][Q1];
+            dn[m + 1] = (sm.d[m][R1 - 1][Q1] + cx0) * (0.5 * dnx) + (sm.d[4 + m][R1 - 1][Q1] + cy0) * (0.5 * dny);
+            up[m + 1] = (cx0 + sm.d[m][R1 + 1][Q1]) * (0.5 * upx) + (cy0 + sm.d[4 + m][R1 + 1][Q1]) * (0.5 * upy);
+            lf[m + 1] = (sm.d[m][R1][Q1 - 1] + cx0) * (0.5 * lfx) + (sm.d[4 + m][R1][Q1 - 1] + cy0) * (0.5 * lfy);
+            rt[m + 1] = (cx0 + sm.d[m][R1][Q1 + 1]) * (0.5 * rtx) + (cy0 + sm.d[4 + m][R1][Q1 + 1]) * (0.5 * rty);
+        }
+        // component 0 of D is identically zero: (0 + 0) * (0.5 n) terms sum to +0
+        dn[0] = 0.0 * (0.5 * dnx) + 0.0 * (0.5 * dny);
+        up[0] = 0.0 * (0.5 * upx) + 0.0 * (0.5 * upy);
+        lf[0] = 0.0 * (0.5 * lfx) + 0.0 * (0.5 * lfy);
+        rt[0] = 0.0 * (0.5 * rtx) + 0.0 * (0.5 * rty);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) fv[m] = up[m] - dn[m] + rt[m] - lf[m];
+
+        // JST dissipation (dissipation.rs:44-185)
+        {
+            const int ri = ti + 2; // sensor_i row of cell i (rows start at i0-2)
+            // tau face i: sensors i-2..i+1, U rows i-2..i+1
+            double m4 = fmax(fmax(fmax(sm.si[ri - 2][tj], sm.si[ri - 1][tj]), sm.si[ri][tj]), sm.si[ri + 1][tj]);
+            double e2 = k.k2 * m4, e4 = fmax(k.k4 - e2, 0.0);
+            double lam = half_cfl * (sm.ratio[R1][Q1] + sm.ratio[R1 - 1][Q1]);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const double um2 = sm.u[m][R - 2][Q], um1 = sm.u[m][R - 1][Q], u0 = sm.u[m][R][Q], up1 = sm.u[m][R + 1][Q];
+                const double d1 = u0 - um1;
+                const double d3 = up1 - u0 * 3.0 + um1 * 3.0 - um2;
+                dn[m] = (d1 * e2 - d3 * e4) * lam;
+            }
+            // tau face i+1: sensors i-1..i+2, U rows i-1..i+2
+            m4 = fmax(fmax(fmax(sm.si[ri - 1][tj], sm.si[ri][tj]), sm.si[ri + 1][tj]), sm.si[ri + 2][tj]);
+            e2 = k.k2 * m4;
+            e4 = fmax(k.k4 - e2, 0.0);
+            lam = half_cfl * (sm.ratio[R1 + 1][Q1] + sm.ratio[R1][Q1]);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const double um2 = sm.u[m][R - 1][Q], um1 = sm.u[m][R][Q], u0 = sm.u[m][R + 1][Q], up1 = sm.u[m][R + 2][Q];
+                const double d1 = u0 - um1;
+                const double d3 = up1 - u0 * 3.0 + um1 * 3.0 - um2;
+                up[m] = (d1 * e2 - d3 * e4) * lam;
+            }
+            const int cj = tj + 2; // sensor_j column of cell j (columns start at j0-2)
+            // n face j: sensors j-2..j+1, U cols j-2..j+1
+            m4 = fmax(fmax(fmax(sm.sj[ti][cj - 2], sm.sj[ti][cj - 1]), sm.sj[ti][cj]), sm.sj[ti][cj + 1]);
+            e2 = k.k2 * m4;
+            e4 = fmax(k.k4 - e2, 0.0);
+            lam = half_cfl * (sm.ratio[R1][Q1] + sm.ratio[R1][Q1 - 1]);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const double um2 = sm.u[m][R][Q - 2], um1 = sm.u[m][R][Q - 1], u0 = sm.u[m][R][Q], up1 = sm.u[m][R][Q + 1];
+                const double d1 = u0 - um1;
+                const double d3 = up1 - u0 * 3.0 + um1 * 3.0 - um2;
+                lf[m] = (d1 * e2 - d3 * e4) * lam;
+            }
+            // n face j+1: sensors j-1..j+2, U cols j-1..j+2
+            m4 = fmax(fmax(fmax(sm.sj[ti][cj - 1], sm.sj[ti][cj]), sm.sj[ti][cj + 1]), sm.sj[ti][cj + 2]);
+            e2 = k.k2 * m4;
+            e4 = fmax(k.k4 - e2, 0.0);
+            lam = half_cfl * (sm.ratio[R1][Q1 + 1] + sm.ratio[R1][Q1]);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const double um2 = sm.u[m][R][Q - 1], um1 = sm.u[m][R][Q], u0 = sm.u[m][R][Q + 1], up1 = sm.u[m][R][Q + 2];
+                const double d1 = u0 - um1;
+                const double d3 = up1 - u0 * 3.0 + um1 * 3.0 - um2;
+                rt[m] = (d1 * e2 - d3 * e4) * lam;
+            }
+#pragma unroll
+            for (int m = 0; m < 5; ++m) fd[m] = up[m] + rt[m] - dn[m] - lf[m];
+        }
+
+        const size_t cc = cidx(i, j, pitch);
+        if (TERMS) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                C.dbg[(DBG_FC + m) * plane + cc] = fc[m];
+                C.dbg[(DBG_FV + m) * plane + cc] = fv[m];
+                C.dbg[(DBG_FD + m) * plane + cc] = fd[m];
+            }
+        } else {
+            // solver.rs:123-145: U = U_former - (Fc - Fv - Fd - S) * (alpha dt / V)
+            double r[5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) r[m] = fc[m] - fv[m] - fd[m];
+            r[4] -= sm.src[ti][tj];
+            const double a_dt = k.lts ? alpha * C.localdt[cc] : alpha * dt;
+            const double sc = a_dt * C.inv_vol[cc];
+            double un[5];
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                un[m] = former[m * plane + cc] - r[m] * sc;
+                dst[m * plane + cc] = un[m];
+            }
+            // solver.rs:151-171: first row-major cell with rho or p NaN or <= 1e-15
+            bool bad = isnan(un[0]) || un[0] <= 1e-15;
+            if (!bad) {
+                const double inv = 1.0 / un[0];
+                const double vx = un[1] * inv, vy = un[2] * inv;
+                const double p = (gamma - 1.0) * (un[3] - un[0] * (vx * vx + vy * vy) * 0.5);
+                bad = isnan(p) || p <= 1e-15;
+            }
+            if (bad) {
+                const unsigned long long key =
+                    ((unsigned long long)seq << 32) | (unsigned long long)((long long)i * nj + j);
+                atomicMin(&ctl->err_key, key);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// End of step (mode 1) or dt-only refresh (mode 0).
+//   mode 1: residual sqrt(sum (rho^{n+1}-rho^n)^2 / NI / NJ) with a deterministic
+//           reduction (per-tile tree, then fixed-order over tiles in the last block),
+//           totaltime += dt, step += 1, rotation advance, convergence / stop control.
+//   both:   localdt and V/dt of the new state (timestep.rs:24-47) and their global minimum
+//           by atomicMin on the bit pattern (order-independent, so deterministic).
+// ---------------------------------------------------------------------------------------
+template <int TI, int TJ, int NT>
+__global__ __launch_bounds__(NT) void post_kernel(const CaseDev *__restrict__ cases,
+                                                  const TileDesc *__restrict__ tiles, int mode)
+{
+    __shared__ double red_s[NT];
+    __shared__ double red_m[NT];
+    __shared__ int last;
+    const TileDesc tl = tiles[blockIdx.x];
+    const CaseDev &C = cases[tl.c];
+    const PhysConst &k = C.k;
+    CaseCtl *ctl = C.ctl;
+    const int ni = C.ni, nj = C.nj, pitch = C.pitch;
+    const size_t plane = (size_t)ni * pitch;
+    const int tid = threadIdx.x;
+    const int status = *(volatile int *)&ctl->status;
+    const unsigned long long ek = *(volatile unsigned long long *)&ctl->err_key;
+    const int rot = *(volatile int *)&ctl->rot;
+    if (mode == 1) {
+        if (status != JST_RUN_ACTIVE) return;
+        if (ek != JST_ERR_NONE) {
+            if (tid == 0) ctl->status = JST_RUN_NONPHYSICAL;
+            return;
+        }
+    }
+    const int b1 = rot == 2 ? 0 : rot + 1;
+    const double *unew = mode == 1 ? C.ubuf[b1] : C.ubuf[rot];
+    const double *uold = C.ubuf[rot];
+    double acc = 0.0, dmin = __builtin_huge_val();
+    for (int o = tid; o < TI * TJ; o += NT) {
+        const int ti = o / TJ, tj = o - ti * TJ;
+        const int i = tl.i0 + ti, j = tl.j0 + tj;
+        if (i >= ni || j >= nj) continue;
+        const size_t c = cidx(i, j, pitch);
+        double u[5];
+        load_u(unew, plane, c, u);
+        if (mode == 1) {
+            const double d = u[0] - uold[c];
+            acc += d * d;
+        }
+        Prim q;
+        unpack(k, u, q);
+        const double cs = sqrt(k.r_gas * k.gamma * q.t);
+        const int jp = j + 1 < nj ? j + 1 : 0;
+        const size_t cu = cidx(i + 1, j, pitch), cr = cidx(i, jp, pitch);
+        const double aa = 0.5 * (C.tnx[c] + C.tnx[cu]);
+        const double bb = 0.5 * (C.tny[c] + C.tny[cu]);
+        const double ccv = 0.5 * (C.nnx[c] + C.nnx[cr]);
+        const double ddv = 0.5 * (C.nny[c] + C.nny[cr]);
+        const double conv = fabs(q.vx * aa + q.vy * bb) + fabs(q.vx * ccv + q.vy * ddv);
+        const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(ccv * ccv + ddv * ddv));
+        const double ldt = k.cfl * C.vol[c] / (conv + acou);
+        C.localdt[c] = ldt;
+        C.ratio[c] = C.vol[c] / ldt;
+        dmin = fmin(dmin, ldt);
+    }
+    red_s[tid] = acc;
+    red_m[tid] = dmin;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            red_s[tid] = red_s[tid] + red_s[tid + s];
+            red_m[tid] = fmin(red_m[tid], red_m[tid + s]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        C.partial[tl.t] = red_s[0];
+        const double m = red_m[0];
+        if (m == m) atomicMin(&ctl->dt_acc, (unsigned long long)__double_as_longlong(m));
+        __threadfence();
+        const unsigned int prev = atomicAdd(&ctl->blocks_done, 1u);
+        last = prev == (unsigned int)(C.ntiles - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // fixed-order reduction of the per-tile partials
+    double s = 0.0;
+    for (int t = tid; t < C.ntiles; t += NT) s += ((volatile double *)C.partial)[t];
+    red_s[tid] = s;
+    __syncthreads();
+    for (int h = NT / 2; h > 0; h >>= 1) {
+        if (tid < h) red_s[tid] = red_s[tid] + red_s[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        volatile CaseCtl *v = ctl;
+        const double new_dt = __longlong_as_double((long long)v->dt_acc);
+        if (mode == 1) {
+            const double res = sqrt(red_s[0] / (double)ni / (double)nj);
+            v->totaltime = v->totaltime + v->dt;
+            const long long st = v->step + 1;
+            v->step = st;
+            v->rot = b1;
+            v->residual = res;
+            C.res_hist[(st - 1) % C.res_cap] = res;
+            if (st == 1) v->res_first = res;
+            bool conv = res < k.targetres;
+            if (k.residual_drop > 0.0 && v->res_first > 0.0)
+                conv = conv || res < v->res_first * pow(10.0, -k.residual_drop);
+            if (!v->timing) {
+                if (conv) v->status = JST_RUN_CONVERGED;
+                else if (st >= v->stop_step) v->status = JST_RUN_STOPPED;
+            }
+        }
+        v->dt = new_dt;
+        v->dt_acc = JST_INF_BITS;
+        v->blocks_done = 0u;
+        __threadfence();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Wall forces: per wall face j, (Fc - Fv)_tau(0,j) from the current state, then a serial
+// FP64 sum in j order by one thread.  out = {Fx, Fy}.
+// ---------------------------------------------------------------------------------------
+__global__ void forces_kernel(const CaseDev *__restrict__ cases, int ci, double *scratch,
+                              double *out)
+{
+    const CaseDev &C = cases[ci];
+    const PhysConst &k = C.k;
+    const int ni = C.ni, nj = C.nj, pitch = C.pitch;
+    const size_t plane = (size_t)ni * pitch;
+    const double *U = C.ubuf[C.ctl->rot];
+    for (int j = threadIdx.x; j < nj; j += blockDim.x) {
+        // prims at (0, j-1), (0, j), (0, j+1), (1, j) and the wall ghost (-1, j)
+        Prim q0, ql, qr, q1, qg;
+        double u0[5], ug[5], tmp[5];
+        const int jm = j > 0 ? j - 1 : nj - 1, jp = j + 1 < nj ? j + 1 : 0;
+        load_u(U, plane, cidx(0, j, pitch), u0);
+        unpack(k, u0, q0);
+        load_u(U, plane, cidx(0, jm, pitch), tmp);
+        unpack(k, tmp, ql);
+        load_u(U, plane, cidx(0, jp, pitch), tmp);
+        unpack(k, tmp, qr);
+        load_u(U, plane, cidx(1, j, pitch), tmp);
+        unpack(k, tmp, q1);
+        qg = q0;
+        qg.vx = -q0.vx;
+        qg.vy = -q0.vy;
+        qg.nut = -q0.nut;
+        pack(qg, ug);
+        const size_t c0 = cidx(0, j, pitch);
+        const double nx = C.tnx[c0], ny = C.tny[c0];
+        double fc[5];
+        euler_flux(ug, u0, nx, ny, k.gamma, fc);
+        // gradient at (0, j)
+        const double upx = C.tnx[cidx(1, j, pitch)], upy = C.tny[cidx(1, j, pitch)];
+        const double lfx = C.nnx[c0], lfy = C.nny[c0];
+        const double rtx = C.nnx[cidx(0, jp, pitch)], rty = C.nny[cidx(0, jp, pitch)];
+        const double inv_v = C.inv_vol[c0];
+        const double cen[4] = {q0.vx, q0.vy, q0.t, q0.nut};
+        const double upv[4] = {q1.vx, q1.vy, q1.t, q1.nut};
+        const double dnv[4] = {qg.vx, qg.vy, qg.t, qg.nut};
+        const double lfv[4] = {ql.vx, ql.vy, ql.t, ql.nut};
+        const double rtv[4] = {qr.vx, qr.vy, qr.t, qr.nut};
+        double g[8];
+        for (int f = 0; f < 4; ++f) {
+            const double f_up = 0.5 * (cen[f] + upv[f]);
+            const double f_dn = 0.5 * (cen[f] + dnv[f]);
+            const double f_lf = 0.5 * (cen[f] + lfv[f]);
+            const double f_rt = 0.5 * (cen[f] + rtv[f]);
+            g[2 * f] = (f_up * upx - f_dn * nx + f_rt * rtx - f_lf * lfx) * inv_v;
+            g[2 * f + 1] = (f_up * upy - f_dn * ny + f_rt * rty - f_lf * lfy) * inv_v;
+        }
+        double d0[8], dg[8], chi, fv1;
+        diffusion_tensor(k, q0.t, u0[4], q0.vx, q0.vy, g, d0, chi, fv1);
+        g[4] = 0.0;
+        g[5] = 0.0;
+        diffusion_tensor(k, qg.t, ug[4], qg.vx, qg.vy, g, dg, chi, fv1);
+        const double fv1x = (dg[0] + d0[0]) * (0.5 * nx) + (dg[4] + d0[4]) * (0.5 * ny);
+        const double fv2y = (dg[1] + d0[1]) * (0.5 * nx) + (dg[5] + d0[5]) * (0.5 * ny);
+        scratch[2 * j] = fc[1] - fv1x;
+        scratch[2 * j + 1] = fc[2] - fv2y;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double fx = 0.0, fy = 0.0;
+        for (int j = 0; j < nj; ++j) {
+            fx -= scratch[2 * j];
+            fy -= scratch[2 * j + 1];
+        }
+        out[0] = fx;
+        out[1] = fy;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------------------
+namespace jst {
+
+hipError_t launch_geometry(const double *xy, int n_theta, int ni, int nj, int pitch,
+                           const GeomPtrs &g, hipStream_t s)
+{
+    dim3 blk(128), grd_c((nj + 127) / 128, ni), grd_f((nj + 127) / 128, ni + 1);
+    hipLaunchKernelGGL(geom_cells_kernel, grd_c, blk, 0, s, xy, n_theta, ni, nj, pitch, g.vol,
+                       g.cx, g.cy);
+    hipLaunchKernelGGL(geom_faces_kernel, grd_f, blk, 0, s, xy, n_theta, ni, nj, pitch, g.tnx,
+                       g.tny, g.tmx, g.tmy, g.nnx, g.nny, g.nmx, g.nmy);
+    hipLaunchKernelGGL(geom_walldist_kernel, grd_c, blk, 0, s, ni, nj, pitch, g.cx, g.cy, g.tmx,
+                       g.tmy, g.vol, g.wd, g.inv_vol, g.inv_wd2);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage_fp64(const CaseDev *cases, const TileDesc *tiles, int ntiles, int stage,
+                             double alpha, bool terms, hipStream_t s)
+{
+    if (terms)
+        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, true>), dim3(ntiles),
+                           dim3(FP64_NT), 0, s, cases, tiles, stage, alpha);
+    else
+        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, false>), dim3(ntiles),
+                           dim3(FP64_NT), 0, s, cases, tiles, stage, alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_post_fp64(const CaseDev *cases, const TileDesc *tiles, int ntiles, int mode,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL((post_kernel<FP64_TI, FP64_TJ, FP64_NT>), dim3(ntiles), dim3(FP64_NT), 0, s,
+                       cases, tiles, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, double *out,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL(forces_kernel, dim3(1), dim3(256), 0, s, cases, ci, scratch, out);
+    return hipGetLastError();
+}
+
+} // namespace jst

@@ -1,0 +1,30 @@
+// jst_launch.h — host-side launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "jst_dev.h"
+
+// FP64 stage tile: TI rows x TJ columns, one cell per thread.
+#define FP64_TI 8
+#define FP64_TJ 32
+#define FP64_NT 256
+
+namespace jst {
+
+struct GeomPtrs {
+    double *vol, *cx, *cy, *wd, *inv_vol, *inv_wd2;
+    double *tnx, *tny, *tmx, *tmy; // (ni+1) x pitch
+    double *nnx, *nny, *nmx, *nmy; // ni x pitch
+};
+
+hipError_t launch_geometry(const double *xy, int n_theta, int ni, int nj, int pitch,
+                           const GeomPtrs &g, hipStream_t s);
+hipError_t launch_stage_fp64(const CaseDev *cases, const TileDesc *tiles, int ntiles, int stage,
+                             double alpha, bool terms, hipStream_t s);
+hipError_t launch_post_fp64(const CaseDev *cases, const TileDesc *tiles, int ntiles, int mode,
+                            hipStream_t s);
+hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, double *out,
+                              hipStream_t s);
+
+} // namespace jst

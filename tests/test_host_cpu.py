@@ -1,0 +1,143 @@
+"""Host logic of libjstgpu.so on the CPU (no GPU touched): exports, config and mesh parsing.
+
+The config checks mirror jst-rs config.rs:297-356; the mesh checks mirror mesh.rs:192-260
+and are cross-checked against the oracle's mesh parser.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, config_dict, config_text, ellipse_mesh, mesh_text
+
+
+@pytest.fixture(scope="module")
+def jst():
+    from purinecfd_amd import jst as mod
+    return mod
+
+
+def test_library_exports_every_declared_symbol(jst):
+    header = (ROOT / "include" / "jst_gpu.h").read_text()
+    declared = re.findall(r"^\s*(?:const\s+char\s*\*|jst_status|void)\s*\**\s*(jst_\w+)\s*\(",
+                          header, re.M)
+    assert set(declared) == set(jst.EXPORTS)
+    lib = C.CDLL(str(jst.LIB_PATH))
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_config_parses_repository_config(jst):
+    cfg = jst.Config.from_str(config_text())
+    d = cfg.derived
+    assert d["halo"] == 3
+    assert abs(d["c_inf"] - math.sqrt(1.4 * 287.06 * 300.0)) < 1e-12
+    assert abs(d["rho_inf"] - 101325.0 / (287.06 * 300.0)) < 1e-12
+    assert abs(d["u_inf"] - d["c_inf"] * 0.2) < 1e-12 and abs(d["v_inf"]) < 1e-12
+    assert d["supersonic"] == 0.0
+
+
+def test_config_derived_matches_oracle_bitwise(jst, oracle_lib):
+    for mutate in ({}, {"AOA": 3.5, "Ma": 0.8}, {"Re": 6e6}, {"AOA": -2.0, "Ma": 1.4}):
+        doc = config_dict()
+        doc["simulation"].update(mutate)
+        got = jst.Config.from_str(json.dumps(doc)).derived
+        want = oracle_lib.derived(oracle_lib.config_from_dict(doc))
+        for key in jst.DERIVED_KEYS[:-1]:
+            w = getattr(want, key)
+            assert got[key] == float(w), (mutate, key, got[key], w)
+
+
+def test_cw1_closure_and_sutherland_reference(jst, oracle_lib):
+    """config.rs:332-355: Cw1 closure; mu(T0) = mu0."""
+    doc = config_dict()
+    d = jst.Config.from_str(json.dumps(doc)).derived
+    sa = doc["spalart_allmaras"]
+    assert abs(d["cw1"] - (sa["Cb1"] / sa["kappa"] ** 2 + (1 + sa["Cb2"]) * sa["sigma"])) < 1e-15
+    c = oracle_lib.config_from_dict(doc)
+    mu = oracle_lib.lib().jo_mu(C.byref(c), C.byref(oracle_lib.derived(c)), doc["physics"]["T0"])
+    assert abs(mu - doc["physics"]["mu0"]) < 1e-20
+
+
+def test_solver_section_is_optional(jst):
+    doc = config_dict()
+    del doc["solver"]
+    jst.Config.from_str(json.dumps(doc))
+
+
+def test_missing_field_is_a_config_error(jst):
+    doc = config_dict()
+    del doc["physics"]["gamma"]
+    with pytest.raises(jst.ConfigError, match="gamma"):
+        jst.Config.from_str(json.dumps(doc))
+    with pytest.raises(jst.ConfigError):
+        jst.Config.from_str("{not json")
+
+
+def test_reynolds_extension_sets_the_reynolds_number(jst):
+    doc = config_dict()
+    doc["simulation"]["Re"] = 1e3
+    d = jst.Config.from_str(json.dumps(doc)).derived
+    re_got = d["rho_inf"] * math.hypot(d["u_inf"], d["v_inf"]) / d["mu_inf"]
+    assert abs(re_got - 1e3) < 1e-9 * 1e3
+
+
+def ring(rings, theta, closed):
+    per = theta + 1 if closed else theta
+    s = f"{rings} {per}\n"
+    for r in range(rings):
+        rad = 1.0 + r
+        for t in range(per):
+            a = 2.0 * math.pi * (t % theta) / theta
+            s += f"{rad * math.cos(a):.10f} {rad * math.sin(a):.10f}\n"
+    return s
+
+
+def test_mesh_parse_open_and_closed(jst):
+    m = jst.Mesh.parse(ring(5, 16, False))
+    assert (m.n_rings, m.n_theta, m.n_cells()) == (5, 16, 64)
+    m = jst.Mesh.parse(ring(5, 16, True))
+    assert (m.n_theta, m.n_cells()) == (16, 64)
+    assert m.node(0, 16) == m.node(0, 0) and m.node(2, 17) == m.node(2, 1)
+
+
+def test_mesh_errors_match_the_reference_variants(jst):
+    with pytest.raises(jst.MeshError) as e:
+        jst.Mesh.parse("not a header\n1.0 2.0\n")
+    assert e.value.kind == "BadHeader"
+    with pytest.raises(jst.MeshError) as e:
+        jst.Mesh.parse("4 8\n1.0 2.0\n")
+    assert e.value.kind == "CountMismatch" and "expected 32 points" in str(e.value)
+    with pytest.raises(jst.MeshError) as e:
+        jst.Mesh.parse(ring(3, 16, False))
+    assert e.value.kind == "TooSmall"
+    with pytest.raises(jst.MeshError) as e:
+        jst.Mesh.parse("2 2\n1.0 zz\n")
+    assert e.value.kind == "BadPoint" and "line 2" in str(e.value)
+    with pytest.raises(jst.MeshError) as e:
+        jst.Mesh.parse("")
+    assert e.value.kind == "Empty"
+    with pytest.raises(jst.MeshError) as e:
+        jst.Mesh.from_path("/nonexistent/mesh.txt")
+    assert e.value.kind == "Io"
+
+
+def test_mesh_parse_matches_oracle(jst, oracle_lib):
+    for text in (mesh_text("fangdata.txt"), mesh_text("cyl_17x40.txt"),
+                 ellipse_mesh(9, 24, closed=True), ellipse_mesh(21, 64, 1.25)):
+        m = jst.Mesh.parse(text)
+        r, t, xy = oracle_lib.parse_mesh(text)
+        assert (m.n_rings, m.n_theta) == (r, t)
+        assert np.array_equal(m.xy, xy)
+
+
+def test_repository_mesh(jst):
+    m = jst.Mesh.parse(mesh_text("fangdata.txt"))
+    assert (m.n_rings, m.n_theta, m.n_cells()) == (10, 12, 108)
+    assert abs(m.node(0, 0)[0] - 1.0) < 1e-12
