@@ -1,25 +1,31 @@
 #!/usr/bin/env python3
 """Throughput benchmark of the JST + SA residual-and-update loop on one MI355X.
 
-A "step" is one explicit RK step (5 fused stage kernels + the step-end reduction kernel)
-over the workload's whole mesh.  The metric is cell-stage updates/s = NI*NJ*5*steps / s,
-with the state resident in HBM before the timed region starts.  One JSON line is printed.
+Default workload (the metric's "FP32-batched vs FP64-plain"): the 64-case uneven C4 batch
+(NACA 4-digit O-meshes 129x65 .. 513x257 drawn from seed 20261203).  A "step" is one RK step
+(5 stage launches + 1 step-end launch) of every case.  Reported:
 
-    python bench.py [--steps K] [--warmup W] [--workload naca0012_257x129] [--no-cpu]
+* value: FP32-mixed batched cell-stage updates/s = sum(NI*NJ)*5*steps / wall seconds, all
+  64 cases resident in HBM and advanced by the same launches (HIP-graph replays), tile shape
+  picked by the autotuner;
+* fp64_plain: the FP64 plain path on the same 64 cases run one case at a time;
+* accuracy: per case, mixed vs FP64 plain after the same number of steps from the same start:
+  max |dCL|, |dCD| and field L2 (the FP64 plain path is pinned to the CPU oracle at 1e-10 per
+  residual evaluation by the test suite);
+* serving: p50/p99 time to completion of a Poisson request stream (mean 20 solves/s) run by
+  the serving loop (jst_serve);
+* roofline: the fused FP32 stage kernel, 152 algorithmic B/cell-stage x cells / its mean
+  launch duration (HIP events around every stage launch on the batch's stream);
+* cpu_baseline: the oracle (C restatement of jst-rs, OpenMP) timed on this host's cores on a
+  bounded sample (one 257x129 case of the batch).
 
-roofline: the dominant kernel is the fused stage kernel; its algorithmic bytes per launch
-are 184 B/cell-stage (FP64: read U^(k-1) 40, read U^n 40, write U^(k) 40, geometry 56,
-frozen V/dt 8) x NI*NJ, divided by its mean launch duration from HIP events recorded on the
-solver's stream around every stage launch.  ``traffic`` comes from the rocprofv3 FETCH_SIZE /
-WRITE_SIZE summary committed under profiles/ (calibrated by a copy kernel), if present.
-cpu_baseline: the C restatement of jst-rs (the oracle, OpenMP row loops) timed on this host
-on a bounded sample of the same workload.
+    python bench.py [--steps K] [--warmup W] [--workload c4|naca0012_257x129|...] [--no-cpu]
 """
 from __future__ import annotations
 
 import argparse
 import json
-import os
+import random
 import re
 import sys
 import time
@@ -29,17 +35,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_FP64 = 184.0
-
-
-def load_workload(name: str) -> dict:
-    from purinecfd_amd import workloads
-    table = {"naca0012_257x129": workloads.c2_naca0012,
-             "cylinder_40x17": workloads.c1_cylinder,
-             "naca0012_1025x513": workloads.c3_naca_large}
-    if name not in table:
-        raise SystemExit(f"unknown workload {name}; choose from {sorted(table)}")
-    return table[name]()
+BYTES = {"fp64": 184.0, "mixed": 152.0}
+METRIC = "cell-stage updates/s FP32-batched vs FP64-plain (ΔCL,ΔCD≤1e-4); p99 solve latency"
 
 
 def profiled_traffic(kernel: str, workload: str):
@@ -51,9 +48,9 @@ def profiled_traffic(kernel: str, workload: str):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(w: dict, budget_s: float = 12.0) -> dict:
+def cpu_baseline(cfg: str, mesh: str, name: str, budget_s: float = 12.0) -> dict:
     from oracle.pyoracle import OracleSolver, lib
-    s = OracleSolver(json.loads(w["config"]), w["mesh"])
+    s = OracleSolver(json.loads(cfg), mesh)
     cells = s.ni * s.nj
     t0 = time.perf_counter()
     s.run(2)
@@ -64,16 +61,152 @@ def cpu_baseline(w: dict, budget_s: float = 12.0) -> dict:
     dt = time.perf_counter() - t0
     return {"value": cells * 5 * n / dt, "unit": "cell-stage updates/s",
             "cores": int(lib().jo_threads()), "kind": "port",
-            "sample": f"{w['name']}: {n} steps of the C restatement of jst-rs (FP64, unfused "
-                      f"kernels, OpenMP row loops) after 2 warm-up steps; {dt:.1f} s"}
+            "sample": f"{name} ({cells} cells): {n} steps of the C restatement of jst-rs (FP64, "
+                      f"unfused kernels, OpenMP row loops) after 2 warm-up steps, {dt:.1f} s"}
+
+
+def fp64_plain_pass(jst, cases, acc_steps: int, time_steps: int):
+    """FP64 plain path one case at a time: CL/CD after acc_steps, then timed steps."""
+    out, cells_steps, secs = [], 0.0, 0.0
+    for w in cases:
+        s = jst.Solver(w["config"], w["mesh"], jst.FP64)
+        s.run(acc_steps)
+        f = s.forces()
+        fields = {n: s.get(n) for n in ("rho", "u", "v", "p")}
+        t0 = time.perf_counter()
+        s.time_steps(time_steps)
+        secs += time.perf_counter() - t0
+        cells_steps += s.n_cells() * 5 * time_steps
+        out.append((f, fields))
+        s.close()
+    return out, cells_steps / secs
+
+
+def mixed_accuracy_pass(jst, cases, acc_steps: int):
+    import numpy as np  # noqa: F401
+    b = jst.Batch(hbm_bytes=8 << 30, max_cases=len(cases))
+    ids = [b.add(w["config"], w["mesh"], acc_steps) for w in cases]
+    b.run(steps_per_round=acc_steps)
+    res = []
+    for w, cid in zip(cases, ids):
+        info = b.info(cid)
+        assert info["step"] == acc_steps, info
+        res.append((b.forces(cid), {n: b.get(cid, n, w["ni"], w["nj"]) for n in ("rho", "u", "v", "p")}))
+    b.close()
+    return res
+
+
+def serving_leg(jst, seconds: float, rate: float, seed: int):
+    from purinecfd_amd import workloads
+    reqs = workloads.c5_requests(seconds=seconds, rate=rate, seed=seed)
+    res, summ = jst.serve([(r["arrival_s"], r["config"], r["mesh"], r["max_steps"]) for r in reqs],
+                          hbm_bytes=16 << 30, max_cases=96, steps_per_round=50, tune=False)
+    conv = sum(1 for r in res if r["status"] == jst.CASE_CONVERGED)
+    summ.update({"requests": len(reqs), "converged": conv, "rate_per_s": rate,
+                 "arrival_window_s": seconds,
+                 "cell_stage_rate": summ["cell_stages"] / max(summ["wall_s"], 1e-9)})
+    return summ
+
+
+def bench_c4(args, jst):
+    import numpy as np
+    from purinecfd_amd import workloads
+    cases = workloads.c4_batch()
+    total = sum(w["ni"] * w["nj"] for w in cases)
+    # accuracy: same start, same number of steps, mixed (batched) vs FP64 plain
+    plain, plain_rate = fp64_plain_pass(jst, cases, args.acc_steps, args.plain_steps)
+    mixed = mixed_accuracy_pass(jst, cases, args.acc_steps)
+    dcl = [abs(m[0]["cl"] - p[0]["cl"]) for m, p in zip(mixed, plain)]
+    dcd = [abs(m[0]["cd"] - p[0]["cd"]) for m, p in zip(mixed, plain)]
+    l2 = {n: max(float(np.linalg.norm(m[1][n] - p[1][n]) / np.linalg.norm(p[1][n]))
+                 for m, p in zip(mixed, plain)) for n in ("rho", "u", "v", "p")}
+    # throughput: every case resident, autotuned tiles, HIP-graph replayed steps
+    b = jst.Batch(hbm_bytes=8 << 30, max_cases=len(cases))
+    for w in cases:
+        b.add(w["config"], w["mesh"], 10**9)
+    variant, grid = b.tune(3)
+    if args.warmup > 0:
+        b.time_steps(args.warmup)
+    t0 = time.perf_counter()
+    ev_ms, _ = b.time_steps(args.steps)
+    wall = time.perf_counter() - t0
+    value = total * 5 * args.steps / wall
+    _, stage_ms = b.time_steps(max(5, min(50, args.steps // 4)), per_stage=True)
+    b.close()
+    achieved = BYTES["mixed"] * total / (stage_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-stage updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic: seeded NACA 4-digit O-meshes, freestream start",
+        "config": {"workload": "C4 batch: 64 uneven NACA 4-digit O-mesh cases 129x65..513x257 "
+                               "(seed 20261203), FP32-mixed batched, FP64 accumulation",
+                   "cases": len(cases), "cells": total, "tile_variant": variant, "grid": grid,
+                   "event_ms_per_step": ev_ms / args.steps},
+        "fp64_plain": {"value": plain_rate, "unit": "cell-stage updates/s",
+                       "how": f"same 64 cases, FP64 plain path, one case at a time, "
+                              f"{args.plain_steps} timed steps each"},
+        "speedup_vs_fp64_plain": value / plain_rate,
+        "accuracy": {"steps": args.acc_steps, "max_abs_dcl": max(dcl), "max_abs_dcd": max(dcd),
+                     "max_field_l2_rel": l2, "cases": len(cases),
+                     "reference": "FP64 plain GPU path (oracle-pinned) at the same step"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": profiled_traffic("mix_stage", "c4_batch64"),
+                     "kernel": "mix_stage_kernel (FP32-mixed, all 64 cases per launch)",
+                     "stage_kernel_ms": stage_ms, "bytes_per_cell_stage": BYTES["mixed"]},
+    }
+    if not args.no_serve:
+        out["serving"] = serving_leg(jst, args.serve_seconds, 20.0, 20261204)
+    if not args.no_cpu:
+        w = next(w for w in cases if w["ni"] == 128)
+        out["cpu_baseline"] = cpu_baseline(w["config"], w["mesh"], w["name"])
+    return out
+
+
+def bench_single(args, jst):
+    from purinecfd_amd import workloads
+    table = {"naca0012_257x129": workloads.c2_naca0012, "cylinder_40x17": workloads.c1_cylinder,
+             "naca0012_1025x513": workloads.c3_naca_large}
+    w = table[args.workload]()
+    prec = jst.MIXED if args.mixed else jst.FP64
+    s = jst.Solver(w["config"], w["mesh"], prec)
+    cells = s.n_cells()
+    if args.warmup > 0:
+        s.time_steps(args.warmup)
+    t0 = time.perf_counter()
+    ev_ms, _ = s.time_steps(args.steps)
+    wall = time.perf_counter() - t0
+    _, stage_ms = s.time_steps(max(10, min(100, args.steps // 4)), per_stage=True)
+    key = "mixed" if args.mixed else "fp64"
+    achieved = BYTES[key] * cells / (stage_ms * 1e-3) / 1e9
+    out = {"metric": METRIC, "value": cells * 5 * args.steps / wall, "unit": "cell-stage updates/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32" if args.mixed else "f64",
+           "data": "synthetic: generated O-mesh, freestream start",
+           "config": {"workload": w["name"], "path": "fp32-mixed" if args.mixed else "fp64-plain",
+                      "cells": cells, "event_ms_per_step": ev_ms / args.steps},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": profiled_traffic("mix_stage" if args.mixed else "stage_fp64", w["name"]),
+                        "stage_kernel_ms": stage_ms, "bytes_per_cell_stage": BYTES[key]}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w["config"], w["mesh"], w["name"])
+    return out
 
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=40)
-    ap.add_argument("--workload", default="naca0012_257x129")
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c4")
+    ap.add_argument("--mixed", action="store_true", help="single-workload mode: FP32-mixed path")
+    ap.add_argument("--acc-steps", type=int, default=100)
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -82,44 +215,7 @@ def main() -> int:
               file=sys.stderr)
         return 2
     from purinecfd_amd import jst
-
-    w = load_workload(args.workload)
-    s = jst.Solver(w["config"], w["mesh"], jst.FP64)
-    cells = s.n_cells()
-    if args.warmup > 0:
-        s.time_steps(args.warmup)
-    t0 = time.perf_counter()
-    ev_ms, _ = s.time_steps(args.steps)
-    wall = time.perf_counter() - t0
-    value = cells * 5 * args.steps / wall
-    # dominant kernel: mean stage-kernel duration from per-launch HIP events (separate pass)
-    nprof = max(10, min(100, args.steps // 4))
-    _, stage_ms = s.time_steps(nprof, per_stage=True)
-    achieved = BYTES_FP64 * cells / (stage_ms * 1e-3) / 1e9
-    traffic = profiled_traffic("stage_fp64", w["name"])
-    out = {
-        "metric": "cell-stage updates/s FP32-batched vs FP64-plain (ΔCL,ΔCD≤1e-4); p99 solve latency",
-        "value": value,
-        "unit": "cell-stage updates/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic: generated O-mesh, freestream initial state",
-        "config": {"workload": w["name"], "path": "fp64-plain", "cells": cells,
-                   "event_ms_per_step": ev_ms / args.steps},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "stage_kernel<8,32,256> (FP64)",
-                     "stage_kernel_ms": stage_ms,
-                     "bytes_per_cell_stage": BYTES_FP64},
-    }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w)
+    out = bench_c4(args, jst) if args.workload == "c4" else bench_single(args, jst)
     print(json.dumps(out))
     return 0
 

@@ -31,7 +31,8 @@ typedef enum {
     JST_ERR_NONPHYSICAL = 7,
     JST_ERR_CONFIG = 8,
     JST_ERR_HIP = 9,
-    JST_ERR_ARG = 10
+    JST_ERR_ARG = 10,
+    JST_ERR_NOSPACE = 11 /* batch: the HBM arena or the case table cannot take the case now */
 } jst_status;
 
 typedef enum {
@@ -138,6 +139,103 @@ jst_status jst_progress(const jst_ctx *ctx, long *step, double *totaltime);
  * stage_ms receives the mean stage-kernel duration.  The device does not stop on
  * convergence during timing. */
 jst_status jst_time_steps(jst_ctx *ctx, int nsteps, double *total_ms, double *stage_ms);
+
+
+/* ======================================================================================
+ * Batched solver and serving loop.  No counterpart in the reference (it solves one case per
+ * process); these are the callers of the same residual-and-update loop that keep many
+ * cases resident in HBM and advance all of them with one kernel launch per RK stage.
+ * ====================================================================================== */
+
+typedef struct jst_batch jst_batch;
+
+/* Per-case run state (device-resident, read back after each synchronised step batch). */
+typedef enum {
+    JST_CASE_RUNNING = 0,
+    JST_CASE_CONVERGED = 1,   /* residual < targetres, or the relative residual_drop */
+    JST_CASE_STOPPED = 2,     /* reached its step limit */
+    JST_CASE_NONPHYSICAL = 3  /* diverged; the other cases continue */
+} jst_case_status;
+
+typedef struct {
+    long step;
+    int status;
+    double residual, res_first, totaltime;
+    long cells;
+} jst_case_info;
+
+/* A batch owns one HIP stream and an HBM arena of hbm_bytes for case state; max_cases is the
+ * size of its case table.  prec must be JST_MIXED. */
+jst_status jst_batch_create(jst_precision prec, size_t hbm_bytes, int max_cases, jst_batch **out);
+void jst_batch_destroy(jst_batch *b);
+/* Device bytes a case occupies (state, geometry, scratch): its admission footprint. */
+jst_status jst_case_footprint(const char *config_json, const char *mesh_text, jst_precision prec,
+                              size_t *bytes);
+/* Admit a case (geometry precompute + freestream start on the device).  max_steps <= 0 uses
+ * the config's solver.iteration.  Returns JST_ERR_NOSPACE when it does not fit now. */
+jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh_text,
+                         long max_steps, int *case_id);
+/* Retire a case and return its memory to the arena. */
+jst_status jst_batch_remove(jst_batch *b, int case_id);
+/* Advance every running case by up to nsteps steps (5 stage launches + 1 step-end launch per
+ * step for the whole batch, replayed from a HIP graph).  Each case stops by itself on
+ * convergence, its step limit or a non-physical state.  sync != 0 waits and refreshes the
+ * per-case info. */
+jst_status jst_batch_step(jst_batch *b, int nsteps, int sync);
+jst_status jst_batch_info(const jst_batch *b, int case_id, jst_case_info *out);
+jst_status jst_batch_forces(jst_batch *b, int case_id, double *cl, double *cd);
+/* Field export for one case (names as jst_get_field: state, geometry, and after
+ * jst_batch_residual_terms the FP32 residual R0..R4 = Fc-Fv-Fd per component and S). */
+jst_status jst_batch_get_field(jst_batch *b, int case_id, const char *name, double *dst,
+                               size_t n);
+jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out);
+/* Residual history of a case: res[k] for steps first..first+n-1 (1-based), within the last
+ * 4096 steps. */
+jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, double *res);
+/* Tile shape (variant index) and grid (0 = one workgroup per tile) of the stage kernel. */
+jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);
+/* Autotuner: times every tile variant and grid size on the current batch for trial_steps
+ * steps each (the steps are real work), keeps the fastest; returns the choice. */
+jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant, int *grid);
+/* Benchmark hook: nsteps steps of every running case with convergence stops disabled;
+ * total_ms from HIP events around the region, stage_ms (may be NULL) the mean duration of
+ * one stage launch from a second pass with events around every stage launch. */
+jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, double *stage_ms);
+jst_status jst_batch_counts(const jst_batch *b, int *active_cases, long *active_cells);
+
+/* One solve request of the serving loop. */
+typedef struct {
+    double arrival_s;         /* seconds after the loop starts */
+    const char *config_json;
+    const char *mesh_text;
+    long max_steps;           /* <= 0: config solver.iteration */
+} jst_request;
+
+typedef struct {
+    double admit_s, done_s;   /* seconds after the loop starts */
+    long steps;
+    int status;               /* jst_case_status at retirement, or -1 if never admitted */
+    double residual, cl, cd;
+} jst_request_result;
+
+typedef struct {
+    size_t hbm_bytes;         /* arena for resident case state */
+    int max_cases;            /* case-table slots (concurrent cases) */
+    int steps_per_round;      /* steps launched between admission/retirement checks */
+    int tune;                 /* run the tile autotuner whenever the batch changes size class */
+} jst_serve_opts;
+
+typedef struct {
+    double wall_s;
+    double cell_stages;       /* sum over cases of NI*NJ*5*steps */
+    double p50_s, p99_s;      /* time to convergence (done - arrival) over completed requests */
+    int completed, nonphysical, max_concurrent;
+} jst_serve_summary;
+
+/* Serving loop: requests are admitted in arrival order as slots and arena memory free up,
+ * run concurrently in one batch, and retired on convergence / step limit / divergence. */
+jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
+                     jst_request_result *results, jst_serve_summary *summary);
 
 #ifdef __cplusplus
 }

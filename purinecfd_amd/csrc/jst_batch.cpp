@@ -1,0 +1,987 @@
+// jst_batch.cpp — batched FP32-mixed driver and the serving loop (include/jst_gpu.h).
+//
+// A batch keeps every admitted case resident in one HBM arena and advances all running cases
+// with one launch per RK stage (plus one step-end launch): the stage kernel walks a tile list
+// spanning every case, so small and large meshes share the GPU.  One step of the whole batch
+// is captured once into a HIP graph and replayed; the case table, tile list and per-case
+// control blocks live in device memory, so the graph stays valid while cases advance,
+// converge or fail independently.  Membership changes rebuild the lists and the graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/jst_gpu.h"
+#include "jst_common.h"
+#include "jst_dev.h"
+#include "jst_launch.h"
+
+using namespace jsth;
+
+#define HIPCHK(x)                                                                            \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            return fail(JST_ERR_HIP, std::string("HIP error: ") + hipGetErrorString(e_) +    \
+                                         " at " #x);                                         \
+    } while (0)
+
+namespace {
+
+const double RK_ALPHA[5] = {0.25, 1.0 / 6.0, 0.375, 0.5, 1.0}; // jst-rs config.rs:295
+
+inline size_t r256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+// First-fit arena over one device allocation; blocks are whole cases.
+struct Arena {
+    char *base = nullptr;
+    size_t size = 0;
+    std::map<size_t, size_t> free_; // offset -> length
+    long long alloc(size_t n)
+    {
+        n = r256(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                size_t off = it->first, len = it->second;
+                free_.erase(it);
+                if (len > n) free_[off + n] = len - n;
+                return (long long)off;
+            }
+        }
+        return -1;
+    }
+    void release(size_t off, size_t n)
+    {
+        n = r256(n);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+// Byte layout of one case inside its arena block.
+struct CaseLayout {
+    size_t u64[2], s32[2], g64[14], g32[7], ratio, localdt, partial, res_hist, posts, scratch, dbg;
+    size_t total;
+};
+
+CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
+{
+    CaseLayout L{};
+    const size_t plane = (size_t)ni * pitch, fplane = (size_t)(ni + 1) * pitch;
+    size_t o = 0;
+    auto carve = [&](size_t b) {
+        size_t a = o;
+        o += r256(b);
+        return a;
+    };
+    for (int b = 0; b < 2; ++b) L.u64[b] = carve(5 * plane * 8);
+    for (int b = 0; b < 2; ++b) L.s32[b] = carve(5 * plane * 4);
+    // FP64 geometry: vol cx cy wd inv_vol inv_wd2 (plane) | tnx tny tmx tmy (fplane) | nnx nny nmx nmy
+    for (int q = 0; q < 6; ++q) L.g64[q] = carve(plane * 8);
+    for (int q = 6; q < 10; ++q) L.g64[q] = carve(fplane * 8);
+    for (int q = 10; q < 14; ++q) L.g64[q] = carve(plane * 8);
+    // FP32 geometry: tnx tny (fplane) | nnx nny vol inv_vol inv_wd2 (plane)
+    for (int q = 0; q < 2; ++q) L.g32[q] = carve(fplane * 4);
+    for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
+    L.ratio = carve(plane * 4);
+    L.localdt = carve(plane * 4);
+    L.partial = carve((size_t)npost * 8);
+    L.res_hist = carve((size_t)res_cap * 8);
+    L.posts = carve((size_t)npost * sizeof(PostDesc));
+    L.scratch = carve((2 * (size_t)nj + 2) * 8);
+    L.dbg = carve(6 * plane * 4);
+    L.total = o;
+    return L;
+}
+
+struct HostCase {
+    int slot = -1;
+    Config cfg{};
+    Derived d{};
+    int ni = 0, nj = 0, pitch = 0, npost = 0;
+    size_t plane = 0, fplane = 0, off = 0, bytes = 0;
+    long max_steps = 0;
+    CaseMix dev{};
+    jst::GeomPtrs g64{};
+    PostDesc *d_posts = nullptr;
+    double *scratch = nullptr;
+    std::vector<PostDesc> posts;
+};
+
+MixConst make_mix_const(const Config &c, const Derived &d)
+{
+    MixConst m{};
+    m.k = make_const(c, d);
+    const double rho = d.rho_inf, u = d.u_inf, v = d.v_inf, p = c.p_inf;
+    const double e = p / (rho * (c.gamma - 1.0)) + 0.5 * (u * u + v * v);
+    m.u64[0] = rho;
+    m.u64[1] = rho * u;
+    m.u64[2] = rho * v;
+    m.u64[3] = rho * e;
+    m.u64[4] = rho * d.nut_inf;
+    m.gamma = (float)c.gamma;
+    m.gm1 = (float)(c.gamma - 1.0);
+    m.r_gas = (float)c.r_gas;
+    m.cfl_half = (float)(0.5 * c.cfl);
+    m.k2 = (float)c.k2;
+    m.k4 = (float)c.k4;
+    m.rho_inf = (float)rho;
+    m.u_inf = (float)u;
+    m.v_inf = (float)v;
+    m.p_inf = (float)p;
+    m.t_inf = (float)c.t_inf;
+    m.m_inf = (float)m.u64[1];
+    m.n_inf = (float)m.u64[2];
+    m.e_inf = (float)m.u64[3];
+    m.h_inf = (float)(m.u64[3] + p);
+    m.rnu_inf = (float)m.u64[4];
+    m.nut_inf = (float)d.nut_inf;
+    m.suth_num = (float)d.suth_num;
+    m.inv_t_ref = (float)d.inv_t_ref;
+    m.t_suth = (float)c.t_suth;
+    m.inv_pr = (float)d.inv_pr;
+    m.inv_prt = (float)d.inv_prt;
+    m.cp = (float)d.cp;
+    m.sigma = (float)c.sigma;
+    m.cv1_cubed = (float)d.cv1_cubed;
+    m.cb1 = (float)c.cb1;
+    m.cb2 = (float)c.cb2;
+    m.ct3 = (float)c.ct3;
+    m.ct4 = (float)c.ct4;
+    m.cw1 = (float)d.cw1;
+    m.cw2 = (float)c.cw2;
+    m.cw3_6 = (float)d.cw3_6;
+    m.one_plus_cw3_6 = (float)d.one_plus_cw3_6;
+    m.fv3 = (float)c.fv3;
+    m.inv_kappa2 = (float)d.inv_kappa2;
+    m.rmax = (float)c.rmax;
+    m.lts = c.lts ? 1 : 0;
+    return m;
+}
+
+} // namespace
+
+struct jst_batch {
+    jst_precision prec = JST_MIXED;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    int max_cases = 0;
+    std::vector<std::unique_ptr<HostCase>> cases; // by slot
+    CaseMix *d_cases = nullptr;
+    CaseCtl *d_ctl = nullptr;
+    CaseCtl *h_ctl = nullptr; // pinned mirror
+    TileDesc *d_tiles = nullptr;
+    size_t tiles_cap = 0;
+    int ntiles = 0;
+    PostDesc *d_posts = nullptr;
+    size_t posts_cap = 0;
+    int nposts = 0;
+    int variant = 0, grid = 0;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool use_graph = true;
+    int res_cap = 4096;
+    double *d_scratch_out = nullptr;
+    CaseDev *d_view = nullptr; // FP64 view of one case for the wall-force kernel
+    hipEvent_t ev[2] = {nullptr, nullptr};
+
+    ~jst_batch()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (void *p : {(void *)arena.base, (void *)d_cases, (void *)d_ctl, (void *)d_tiles,
+                        (void *)d_posts, (void *)d_scratch_out, (void *)d_view})
+            if (p) (void)hipFree(p);
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+jst_status drop_graph(jst_batch *b)
+{
+    if (b->gexec) HIPCHK(hipGraphExecDestroy(b->gexec));
+    if (b->graph) HIPCHK(hipGraphDestroy(b->graph));
+    b->gexec = nullptr;
+    b->graph = nullptr;
+    return JST_OK;
+}
+
+// Rebuild the tile list (XCD-aware: consecutive tiles of a case are dealt to the same XCD,
+// since workgroup w runs on XCD w % 8) and the step-end work list.
+jst_status rebuild_lists(jst_batch *b)
+{
+    const jst::MixVariant v = jst::MIX_VARIANTS[b->variant];
+    std::vector<TileDesc> nat;
+    std::vector<PostDesc> posts;
+    for (auto &hc : b->cases) {
+        if (!hc) continue;
+        const int nti = (hc->ni + v.ti - 1) / v.ti, ntj = (hc->nj + v.tj - 1) / v.tj;
+        for (int a = 0; a < nti; ++a)
+            for (int c = 0; c < ntj; ++c) nat.push_back(TileDesc{hc->slot, a * v.ti, c * v.tj, a * ntj + c});
+        posts.insert(posts.end(), hc->posts.begin(), hc->posts.end());
+    }
+    const int n = (int)nat.size(), per = (n + 7) / 8;
+    std::vector<TileDesc> tiles;
+    tiles.reserve(n);
+    for (int q = 0; q < per; ++q)
+        for (int x = 0; x < 8; ++x) {
+            int t = x * per + q;
+            if (t < n) tiles.push_back(nat[t]);
+        }
+    if (tiles.size() > b->tiles_cap) {
+        if (b->d_tiles) HIPCHK(hipFree(b->d_tiles));
+        b->tiles_cap = std::max<size_t>(tiles.size() * 2, 1024);
+        HIPCHK(hipMalloc(&b->d_tiles, b->tiles_cap * sizeof(TileDesc)));
+    }
+    if (posts.size() > b->posts_cap) {
+        if (b->d_posts) HIPCHK(hipFree(b->d_posts));
+        b->posts_cap = std::max<size_t>(posts.size() * 2, 1024);
+        HIPCHK(hipMalloc(&b->d_posts, b->posts_cap * sizeof(PostDesc)));
+    }
+    if (!tiles.empty())
+        HIPCHK(hipMemcpyAsync(b->d_tiles, tiles.data(), tiles.size() * sizeof(TileDesc),
+                              hipMemcpyHostToDevice, b->stream));
+    if (!posts.empty())
+        HIPCHK(hipMemcpyAsync(b->d_posts, posts.data(), posts.size() * sizeof(PostDesc),
+                              hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->ntiles = (int)tiles.size();
+    b->nposts = (int)posts.size();
+    jst_status st = drop_graph(b);
+    if (st != JST_OK) return st;
+    b->dirty = false;
+    return JST_OK;
+}
+
+jst_status launch_one_step(jst_batch *b)
+{
+    for (int k = 0; k < 5; ++k)
+        HIPCHK(jst::launch_stage_mixed(b->variant, b->d_cases, b->d_tiles, b->ntiles, b->grid, k,
+                                       RK_ALPHA[k], false, b->stream));
+    HIPCHK(jst::launch_post_mixed(b->d_cases, b->d_posts, b->nposts, 1, b->stream));
+    return JST_OK;
+}
+
+jst_status launch_steps(jst_batch *b, int nsteps)
+{
+    if (b->dirty) {
+        jst_status st = rebuild_lists(b);
+        if (st != JST_OK) return st;
+    }
+    if (b->ntiles == 0) return JST_OK;
+    if (b->use_graph && !b->gexec) {
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        jst_status st = launch_one_step(b);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (st != JST_OK) return st;
+        HIPCHK(e);
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        if (b->gexec) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
+        else {
+            jst_status st = launch_one_step(b);
+            if (st != JST_OK) return st;
+        }
+    }
+    return JST_OK;
+}
+
+jst_status fetch_ctl(jst_batch *b)
+{
+    HIPCHK(hipMemcpyAsync(b->h_ctl, b->d_ctl, sizeof(CaseCtl) * (size_t)b->max_cases,
+                          hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return JST_OK;
+}
+
+template <class T> jst_status set_ctl_field(jst_batch *b, int slot, T CaseCtl::*field, T value)
+{
+    CaseCtl *h = &b->h_ctl[slot];
+    h->*field = value;
+    const size_t off = (size_t)((char *)&(h->*field) - (char *)h);
+    HIPCHK(hipMemcpyAsync((char *)&b->d_ctl[slot] + off, &(h->*field), sizeof(T),
+                          hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return JST_OK;
+}
+
+HostCase *get_case(const jst_batch *b, int id)
+{
+    if (!b || id < 0 || id >= (int)b->cases.size()) return nullptr;
+    return b->cases[id].get();
+}
+
+jst_status download(jst_batch *b, const void *d, size_t bytes, void *h)
+{
+    HIPCHK(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return JST_OK;
+}
+
+} // namespace
+
+
+namespace jsth {
+
+// Wall force {Fx, Fy, CL, CD} of one case: the FP64 wall-force kernel runs on an FP64 view
+// of the case (its U^n buffers and FP64 geometry): forces are an FP64 reduction.
+jst_status batch_forces(jst_batch *b, int case_id, double out[4])
+{
+    HostCase *hc = get_case(b, case_id);
+    if (!hc) return fail(JST_ERR_ARG, "no such case");
+    CaseDev v{};
+    v.ni = hc->ni;
+    v.nj = hc->nj;
+    v.pitch = hc->pitch;
+    v.ubuf[0] = hc->dev.u64[0];
+    v.ubuf[1] = hc->dev.u64[1];
+    v.tnx = hc->g64.tnx;
+    v.tny = hc->g64.tny;
+    v.tmx = hc->g64.tmx;
+    v.tmy = hc->g64.tmy;
+    v.nnx = hc->g64.nnx;
+    v.nny = hc->g64.nny;
+    v.vol = hc->g64.vol;
+    v.inv_vol = hc->g64.inv_vol;
+    v.inv_wd2 = hc->g64.inv_wd2;
+    v.cx = hc->g64.cx;
+    v.cy = hc->g64.cy;
+    v.ctl = b->d_ctl + case_id;
+    v.k = hc->dev.k.k;
+    HIPCHK(hipMemcpyAsync(b->d_view, &v, sizeof(CaseDev), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(jst::launch_forces_fp64(b->d_view, 0, hc->scratch, b->d_scratch_out, b->stream));
+    double f[2];
+    jst_status st = download(b, b->d_scratch_out, sizeof f, f);
+    if (st != JST_OK) return st;
+    const Config &c = hc->cfg;
+    const double aoa = c.aoa * (M_PI / 180.0), ca = std::cos(aoa), sa = std::sin(aoa);
+    const double q = 0.5 * hc->d.rho_inf * (hc->d.u_inf * hc->d.u_inf + hc->d.v_inf * hc->d.v_inf);
+    const double ql = q * c.ref_length;
+    out[0] = f[0];
+    out[1] = f[1];
+    out[2] = (-f[0] * sa + f[1] * ca) / ql;
+    out[3] = (f[0] * ca + f[1] * sa) / ql;
+    return JST_OK;
+}
+
+// Solver::run (jst-rs solver.rs:203-226) for one case of a batch: the device stops by itself
+// on convergence or after `limit` steps (unless no_stop, the advance() semantics); residuals
+// are read back per synchronised batch of steps and fed to cb in order.
+jst_status batch_run_case(jst_batch *b, int id, long limit, jst_step_cb cb, void *user,
+                          jst_report *out, bool no_stop)
+{
+    HostCase *hc = get_case(b, id);
+    if (!hc) return fail(JST_ERR_ARG, "no such case");
+    jst_status st = fetch_ctl(b);
+    if (st != JST_OK) return st;
+    if (b->h_ctl[id].err_key != JST_ERR_NONE) return fail(JST_ERR_NONPHYSICAL, "non-physical state");
+    const long long start = b->h_ctl[id].step;
+    if ((st = set_ctl_field(b, id, &CaseCtl::timing, no_stop ? 1 : 0)) != JST_OK) return st;
+    if ((st = set_ctl_field(b, id, &CaseCtl::stop_step, (long long)(start + limit))) != JST_OK) return st;
+    if ((st = set_ctl_field(b, id, &CaseCtl::status, (int)JST_RUN_ACTIVE)) != JST_OK) return st;
+    long done = 0;
+    int batch = 8;
+    std::vector<double> hist((size_t)b->res_cap);
+    while (done < limit) {
+        const int n = (int)std::min<long>(batch, limit - done);
+        if ((st = launch_steps(b, n)) != JST_OK) return st;
+        if ((st = fetch_ctl(b)) != JST_OK) return st;
+        const CaseCtl &c = b->h_ctl[id];
+        const long nd = (long)(c.step - start) - done;
+        if (nd > 0) {
+            if ((st = download(b, hc->dev.res_hist, hist.size() * 8, hist.data())) != JST_OK) return st;
+            for (long q = 0; q < nd; ++q)
+                if (cb) cb(done + q + 1, hist[(size_t)((start + done + q) % b->res_cap)], user);
+            done += nd;
+        }
+        if (c.err_key != JST_ERR_NONE) {
+            if (out) *out = jst_report{done, c.residual, c.totaltime, 0};
+            const long long cell = (long long)(c.err_key & 0xFFFFFFFFull);
+            char buf[160];
+            snprintf(buf, sizeof buf, "non-physical state at cell (%lld, %lld)", cell / hc->nj, cell % hc->nj);
+            set_ctl_field(b, id, &CaseCtl::timing, 0);
+            return fail(JST_ERR_NONPHYSICAL, buf);
+        }
+        if (c.status != JST_RUN_ACTIVE) break;
+        batch = std::min(batch * 2, std::min(512, b->res_cap));
+    }
+    if ((st = set_ctl_field(b, id, &CaseCtl::timing, 0)) != JST_OK) return st;
+    const CaseCtl &c = b->h_ctl[id];
+    if (out) {
+        out->steps = done;
+        out->residual = done > 0 ? c.residual : NAN;
+        out->totaltime = c.totaltime;
+        out->converged = c.status == JST_RUN_CONVERGED ? 1 : 0;
+    }
+    return JST_OK;
+}
+
+} // namespace jsth
+
+extern "C" {
+
+jst_status jst_case_footprint(const char *config_json, const char *mesh_text, jst_precision prec,
+                              size_t *bytes)
+{
+    if (!bytes) return fail(JST_ERR_ARG, "bytes is NULL");
+    Config c;
+    Mesh m;
+    jst_status st = parse_config(config_json, c);
+    if (st != JST_OK) return st;
+    st = parse_mesh(mesh_text, m);
+    if (st != JST_OK) return st;
+    if (prec != JST_MIXED) return fail(JST_ERR_ARG, "batches run the FP32-mixed path");
+    const int ni = m.n_rings - 1, nj = m.n_theta, pitch = (nj + 31) / 32 * 32;
+    const int npost = (int)(((size_t)ni * nj + JST_POST_CELLS - 1) / JST_POST_CELLS);
+    *bytes = layout_for(ni, nj, pitch, npost, 4096).total + r256(m.xy.size() * 8);
+    return JST_OK;
+}
+
+jst_status jst_batch_create(jst_precision prec, size_t hbm_bytes, int max_cases, jst_batch **out)
+{
+    if (!out || max_cases <= 0 || hbm_bytes == 0) return fail(JST_ERR_ARG, "bad arguments");
+    if (prec != JST_MIXED) return fail(JST_ERR_ARG, "batches run the FP32-mixed path");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(JST_ERR_HIP, "no HIP device available");
+    std::unique_ptr<jst_batch> b(new jst_batch());
+    b->prec = prec;
+    b->max_cases = max_cases;
+    b->cases.resize(max_cases);
+    HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&b->ev[0]));
+    HIPCHK(hipEventCreate(&b->ev[1]));
+    HIPCHK(hipMalloc((void **)&b->arena.base, hbm_bytes));
+    b->arena.size = hbm_bytes;
+    b->arena.free_[0] = hbm_bytes;
+    HIPCHK(hipMalloc((void **)&b->d_cases, sizeof(CaseMix) * (size_t)max_cases));
+    HIPCHK(hipMalloc((void **)&b->d_ctl, sizeof(CaseCtl) * (size_t)max_cases));
+    HIPCHK(hipMalloc((void **)&b->d_scratch_out, 64));
+    HIPCHK(hipMalloc((void **)&b->d_view, sizeof(CaseDev)));
+    HIPCHK(hipHostMalloc((void **)&b->h_ctl, sizeof(CaseCtl) * (size_t)max_cases, hipHostMallocDefault));
+    memset(b->h_ctl, 0, sizeof(CaseCtl) * (size_t)max_cases);
+    for (int q = 0; q < max_cases; ++q) {
+        b->h_ctl[q].status = JST_RUN_STOPPED;
+        b->h_ctl[q].err_key = JST_ERR_NONE;
+        b->h_ctl[q].dt_acc = JST_INF_BITS;
+    }
+    HIPCHK(hipMemcpy(b->d_ctl, b->h_ctl, sizeof(CaseCtl) * (size_t)max_cases, hipMemcpyHostToDevice));
+    *out = b.release();
+    return JST_OK;
+}
+
+void jst_batch_destroy(jst_batch *b) { delete b; }
+
+jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh_text,
+                         long max_steps, int *case_id)
+{
+    if (!b || !case_id) return fail(JST_ERR_ARG, "NULL argument");
+    std::unique_ptr<HostCase> hc(new HostCase());
+    Mesh mesh;
+    jst_status st = parse_config(config_json, hc->cfg);
+    if (st != JST_OK) return st;
+    st = parse_mesh(mesh_text, mesh);
+    if (st != JST_OK) return st;
+    if (hc->cfg.halo != JST_H) return fail(JST_ERR_CONFIG, "unsupported halo: kernels are built for IM=3");
+    int slot = -1;
+    for (int q = 0; q < b->max_cases; ++q)
+        if (!b->cases[q]) {
+            slot = q;
+            break;
+        }
+    if (slot < 0) return fail(JST_ERR_NOSPACE, "case table full");
+    hc->slot = slot;
+    hc->d = finish(hc->cfg);
+    hc->ni = mesh.n_rings - 1;
+    hc->nj = mesh.n_theta;
+    hc->pitch = (hc->nj + 31) / 32 * 32;
+    hc->plane = (size_t)hc->ni * hc->pitch;
+    hc->fplane = (size_t)(hc->ni + 1) * hc->pitch;
+    hc->npost = (int)(((size_t)hc->ni * hc->nj + JST_POST_CELLS - 1) / JST_POST_CELLS);
+    hc->max_steps = max_steps > 0 ? max_steps : hc->cfg.iteration;
+    const CaseLayout L = layout_for(hc->ni, hc->nj, hc->pitch, hc->npost, b->res_cap);
+    const size_t xy_bytes = r256(mesh.xy.size() * 8);
+    long long off = b->arena.alloc(L.total + xy_bytes);
+    if (off < 0) return fail(JST_ERR_NOSPACE, "HBM arena cannot hold the case now");
+    hc->off = (size_t)off;
+    hc->bytes = L.total + xy_bytes;
+    char *base = b->arena.base + off;
+    auto D = [&](size_t o) { return (double *)(base + o); };
+    auto F = [&](size_t o) { return (float *)(base + o); };
+    CaseMix &cm = hc->dev;
+    cm.ni = hc->ni;
+    cm.nj = hc->nj;
+    cm.pitch = hc->pitch;
+    cm.npost = hc->npost;
+    for (int q = 0; q < 2; ++q) {
+        cm.u64[q] = D(L.u64[q]);
+        cm.s32[q] = F(L.s32[q]);
+    }
+    jst::GeomPtrs &g = hc->g64;
+    double **g64p[14] = {&g.vol, &g.cx, &g.cy, &g.wd, &g.inv_vol, &g.inv_wd2, &g.tnx, &g.tny,
+                         &g.tmx, &g.tmy, &g.nnx, &g.nny, &g.nmx, &g.nmy};
+    for (int q = 0; q < 14; ++q) *g64p[q] = D(L.g64[q]);
+    float *g32[7];
+    for (int q = 0; q < 7; ++q) g32[q] = F(L.g32[q]);
+    cm.tnx = g32[0];
+    cm.tny = g32[1];
+    cm.nnx = g32[2];
+    cm.nny = g32[3];
+    cm.vol = g32[4];
+    cm.inv_vol = g32[5];
+    cm.inv_wd2 = g32[6];
+    cm.tnx64 = g.tnx;
+    cm.tny64 = g.tny;
+    cm.tmx64 = g.tmx;
+    cm.tmy64 = g.tmy;
+    cm.cx64 = g.cx;
+    cm.cy64 = g.cy;
+    cm.ratio = F(L.ratio);
+    cm.localdt = F(L.localdt);
+    cm.partial = D(L.partial);
+    cm.res_hist = D(L.res_hist);
+    cm.res_cap = b->res_cap;
+    cm.ctl = b->d_ctl + slot;
+    cm.dbg = F(L.dbg);
+    cm.k = make_mix_const(hc->cfg, hc->d);
+    hc->d_posts = (PostDesc *)(base + L.posts);
+    hc->scratch = D(L.scratch);
+    for (int p = 0; p < hc->npost; ++p) {
+        const int first = p * JST_POST_CELLS;
+        const int count = std::min(JST_POST_CELLS, hc->ni * hc->nj - first);
+        hc->posts.push_back(PostDesc{slot, first, count, p});
+    }
+    hipStream_t s = b->stream;
+    // zero the block (ghost-free planes, pitch padding), geometry in FP64, FP32 copies
+    HIPCHK(hipMemsetAsync(base, 0, L.total, s));
+    double *d_xy = D(L.total);
+    HIPCHK(hipMemcpyAsync(d_xy, mesh.xy.data(), mesh.xy.size() * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(jst::launch_geometry(d_xy, mesh.n_theta, hc->ni, hc->nj, hc->pitch, g, s));
+    HIPCHK(jst::launch_to_f32(g.tnx, g32[0], hc->fplane, s));
+    HIPCHK(jst::launch_to_f32(g.tny, g32[1], hc->fplane, s));
+    HIPCHK(jst::launch_to_f32(g.nnx, g32[2], hc->plane, s));
+    HIPCHK(jst::launch_to_f32(g.nny, g32[3], hc->plane, s));
+    HIPCHK(jst::launch_to_f32(g.vol, g32[4], hc->plane, s));
+    HIPCHK(jst::launch_to_f32(g.inv_vol, g32[5], hc->plane, s));
+    HIPCHK(jst::launch_to_f32(g.inv_wd2, g32[6], hc->plane, s));
+    HIPCHK(hipMemcpyAsync(hc->d_posts, hc->posts.data(), hc->posts.size() * sizeof(PostDesc),
+                          hipMemcpyHostToDevice, s));
+    // freestream start (jst-rs state.rs:174-190), packed
+    {
+        std::vector<double> h(5 * hc->plane, 0.0);
+        for (int m = 0; m < 5; ++m)
+            for (int i = 0; i < hc->ni; ++i)
+                for (int j = 0; j < hc->nj; ++j) h[m * hc->plane + (size_t)i * hc->pitch + j] = cm.k.u64[m];
+        HIPCHK(hipMemcpyAsync(cm.u64[0], h.data(), h.size() * 8, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    CaseCtl c0{};
+    c0.dt_acc = JST_INF_BITS;
+    c0.residual = NAN;
+    c0.err_key = JST_ERR_NONE;
+    c0.status = JST_RUN_ACTIVE;
+    c0.stop_step = hc->max_steps;
+    b->h_ctl[slot] = c0;
+    HIPCHK(hipMemcpyAsync(b->d_ctl + slot, &b->h_ctl[slot], sizeof(CaseCtl), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(b->d_cases + slot, &cm, sizeof(CaseMix), hipMemcpyHostToDevice, s));
+    HIPCHK(jst::launch_shadow(b->d_cases, slot, 5 * hc->plane, s));
+    HIPCHK(jst::launch_post_mixed(b->d_cases, hc->d_posts, hc->npost, 0, s));
+    HIPCHK(hipStreamSynchronize(s));
+    // the node buffer is only needed for the geometry precompute
+    b->arena.release(hc->off + L.total, xy_bytes);
+    hc->bytes = L.total;
+    b->cases[slot] = std::move(hc);
+    b->dirty = true;
+    *case_id = slot;
+    return JST_OK;
+}
+
+jst_status jst_batch_remove(jst_batch *b, int case_id)
+{
+    HostCase *hc = get_case(b, case_id);
+    if (!hc) return fail(JST_ERR_ARG, "no such case");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    jst_status st = set_ctl_field(b, case_id, &CaseCtl::status, (int)JST_RUN_STOPPED);
+    if (st != JST_OK) return st;
+    b->arena.release(hc->off, hc->bytes);
+    b->cases[case_id].reset();
+    b->dirty = true;
+    return JST_OK;
+}
+
+jst_status jst_batch_step(jst_batch *b, int nsteps, int sync)
+{
+    if (!b || nsteps < 0) return fail(JST_ERR_ARG, "bad arguments");
+    jst_status st = launch_steps(b, nsteps);
+    if (st != JST_OK) return st;
+    return sync ? fetch_ctl(b) : JST_OK;
+}
+
+jst_status jst_batch_info(const jst_batch *b, int case_id, jst_case_info *out)
+{
+    HostCase *hc = get_case(b, case_id);
+    if (!hc || !out) return fail(JST_ERR_ARG, "no such case");
+    const CaseCtl &c = b->h_ctl[case_id];
+    out->step = (long)c.step;
+    out->status = c.err_key != JST_ERR_NONE ? JST_CASE_NONPHYSICAL : c.status;
+    out->residual = c.residual;
+    out->res_first = c.res_first;
+    out->totaltime = c.totaltime;
+    out->cells = (long)hc->ni * hc->nj;
+    return JST_OK;
+}
+
+jst_status jst_batch_counts(const jst_batch *b, int *active_cases, long *active_cells)
+{
+    if (!b) return fail(JST_ERR_ARG, "NULL batch");
+    int n = 0;
+    long cells = 0;
+    for (auto &hc : b->cases)
+        if (hc && b->h_ctl[hc->slot].status == JST_RUN_ACTIVE && b->h_ctl[hc->slot].err_key == JST_ERR_NONE) {
+            ++n;
+            cells += (long)hc->ni * hc->nj;
+        }
+    if (active_cases) *active_cases = n;
+    if (active_cells) *active_cells = cells;
+    return JST_OK;
+}
+
+jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, double *res)
+{
+    HostCase *hc = get_case(b, case_id);
+    if (!hc || !res || first < 1 || n < 0) return fail(JST_ERR_ARG, "bad arguments");
+    std::vector<double> h((size_t)b->res_cap);
+    jst_status st = download(b, hc->dev.res_hist, h.size() * 8, h.data());
+    if (st != JST_OK) return st;
+    for (long q = 0; q < n; ++q) res[q] = h[(size_t)((first - 1 + q) % b->res_cap)];
+    return JST_OK;
+}
+
+jst_status jst_batch_forces(jst_batch *b, int case_id, double *cl, double *cd)
+{
+    double f[4];
+    jst_status st = jsth::batch_forces(b, case_id, f);
+    if (st != JST_OK) return st;
+    if (cl) *cl = f[2];
+    if (cd) *cd = f[3];
+    return JST_OK;
+}
+
+jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out)
+{
+    HostCase *hc = get_case(b, case_id);
+    if (!hc) return fail(JST_ERR_ARG, "no such case");
+    const jst::MixVariant v = jst::MIX_VARIANTS[b->variant];
+    std::vector<TileDesc> t;
+    const int nti = (hc->ni + v.ti - 1) / v.ti, ntj = (hc->nj + v.tj - 1) / v.tj;
+    for (int a = 0; a < nti; ++a)
+        for (int c = 0; c < ntj; ++c) t.push_back(TileDesc{case_id, a * v.ti, c * v.tj, a * ntj + c});
+    TileDesc *dt = nullptr;
+    HIPCHK(hipMalloc(&dt, t.size() * sizeof(TileDesc)));
+    HIPCHK(hipMemcpyAsync(dt, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(jst::launch_post_mixed(b->d_cases, hc->d_posts, hc->npost, 0, b->stream));
+    HIPCHK(jst::launch_stage_mixed(b->variant, b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true,
+                                   b->stream));
+    jst_status st = fetch_ctl(b);
+    HIPCHK(hipFree(dt));
+    if (st != JST_OK) return st;
+    if (dt_out) *dt_out = b->h_ctl[case_id].dt;
+    return JST_OK;
+}
+
+jst_status jst_batch_get_field(jst_batch *b, int case_id, const char *name, double *dst, size_t n)
+{
+    HostCase *hc = get_case(b, case_id);
+    if (!hc || !name || !dst) return fail(JST_ERR_ARG, "bad arguments");
+    const int ni = hc->ni, nj = hc->nj, pitch = hc->pitch;
+    std::string s(name);
+    jst_status st = fetch_ctl(b);
+    if (st != JST_OK) return st;
+    const jst::GeomPtrs &g = hc->g64;
+    struct G {
+        const char *n;
+        const double *p;
+        bool face;
+    } geo[] = {{"vol", g.vol, false}, {"cx", g.cx, false}, {"cy", g.cy, false},
+               {"wall_dist", g.wd, false}, {"inv_vol", g.inv_vol, false}, {"inv_wd2", g.inv_wd2, false},
+               {"tau_nx", g.tnx, true}, {"tau_ny", g.tny, true}, {"tau_mx", g.tmx, true},
+               {"tau_my", g.tmy, true}, {"n_nx", g.nnx, false}, {"n_ny", g.nny, false},
+               {"n_mx", g.nmx, false}, {"n_my", g.nmy, false}};
+    for (auto &gd : geo)
+        if (s == gd.n) {
+            const int rows = gd.face ? ni + 1 : ni;
+            if (n < (size_t)rows * nj) return fail(JST_ERR_ARG, "destination too small");
+            std::vector<double> h((size_t)rows * pitch);
+            if ((st = download(b, gd.p, h.size() * 8, h.data())) != JST_OK) return st;
+            for (int i = 0; i < rows; ++i)
+                for (int j = 0; j < nj; ++j) dst[(size_t)i * nj + j] = h[(size_t)i * pitch + j];
+            return JST_OK;
+        }
+    if (n < (size_t)ni * nj) return fail(JST_ERR_ARG, "destination too small");
+    auto plane32 = [&](const float *p) -> jst_status {
+        std::vector<float> h(hc->plane);
+        jst_status q = download(b, p, h.size() * 4, h.data());
+        if (q != JST_OK) return q;
+        for (int i = 0; i < ni; ++i)
+            for (int j = 0; j < nj; ++j) dst[(size_t)i * nj + j] = h[(size_t)i * pitch + j];
+        return JST_OK;
+    };
+    if (s.size() == 2 && s[0] == 'R' && s[1] >= '0' && s[1] <= '4')
+        return plane32(hc->dev.dbg + (size_t)(s[1] - '0') * hc->plane);
+    if (s == "S") return plane32(hc->dev.dbg + 5 * hc->plane);
+    if (s == "localdt") return plane32(hc->dev.localdt);
+    if (s == "ratio") return plane32(hc->dev.ratio);
+    std::vector<double> h(5 * hc->plane);
+    if ((st = download(b, hc->dev.u64[b->h_ctl[case_id].rot], h.size() * 8, h.data())) != JST_OK)
+        return st;
+    if (s.size() == 2 && s[0] == 'U' && s[1] >= '0' && s[1] <= '4') {
+        const size_t base = (size_t)(s[1] - '0') * hc->plane;
+        for (int i = 0; i < ni; ++i)
+            for (int j = 0; j < nj; ++j) dst[(size_t)i * nj + j] = h[base + (size_t)i * pitch + j];
+        return JST_OK;
+    }
+    static const char *prims[] = {"rho", "p", "T", "u", "v", "E", "H", "c", "nut"};
+    int which = -1;
+    for (int q = 0; q < 9; ++q)
+        if (s == prims[q]) which = q;
+    if (which < 0) return fail(JST_ERR_ARG, "unknown field " + s);
+    const Config &c = hc->cfg;
+    for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj; ++j) {
+            double u[5];
+            for (int m = 0; m < 5; ++m) u[m] = h[m * hc->plane + (size_t)i * pitch + j];
+            const double rho = u[0], inv = 1.0 / rho, vx = u[1] * inv, vy = u[2] * inv, e = u[3] * inv;
+            const double p = (c.gamma - 1.0) * (u[3] - rho * (vx * vx + vy * vy) * 0.5);
+            const double t = p / (c.r_gas * rho);
+            const double v[9] = {rho, p, t, vx, vy, e, e + p * inv, std::sqrt(c.r_gas * c.gamma * t), u[4] * inv};
+            dst[(size_t)i * nj + j] = v[which];
+        }
+    return JST_OK;
+}
+
+jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid)
+{
+    if (!b || variant < 0 || variant >= MIX_NVARIANTS || grid < 0) return fail(JST_ERR_ARG, "bad tiling");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (variant != b->variant) b->dirty = true;
+    if (grid != b->grid) {
+        jst_status st = drop_graph(b);
+        if (st != JST_OK) return st;
+    }
+    b->variant = variant;
+    b->grid = grid;
+    return JST_OK;
+}
+
+jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, double *stage_ms)
+{
+    if (!b || nsteps <= 0) return fail(JST_ERR_ARG, "bad arguments");
+    jst_status st = fetch_ctl(b);
+    if (st != JST_OK) return st;
+    std::vector<int> touched;
+    for (auto &hc : b->cases)
+        if (hc && b->h_ctl[hc->slot].status == JST_RUN_ACTIVE) {
+            if ((st = set_ctl_field(b, hc->slot, &CaseCtl::timing, 1)) != JST_OK) return st;
+            touched.push_back(hc->slot);
+        }
+    if (b->dirty && (st = rebuild_lists(b)) != JST_OK) return st;
+    // build (and warm) the graph outside the timed region
+    if ((st = launch_steps(b, 1)) != JST_OK) return st;
+    HIPCHK(hipEventRecord(b->ev[0], b->stream));
+    if ((st = launch_steps(b, nsteps)) != JST_OK) return st;
+    HIPCHK(hipEventRecord(b->ev[1], b->stream));
+    HIPCHK(hipEventSynchronize(b->ev[1]));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+    if (total_ms) *total_ms = ms;
+    if (stage_ms) {
+        std::vector<hipEvent_t> evs(10 * (size_t)nsteps);
+        for (auto &e : evs) HIPCHK(hipEventCreate(&e));
+        for (int s = 0; s < nsteps; ++s) {
+            for (int k = 0; k < 5; ++k) {
+                HIPCHK(hipEventRecord(evs[10 * s + 2 * k], b->stream));
+                HIPCHK(jst::launch_stage_mixed(b->variant, b->d_cases, b->d_tiles, b->ntiles, b->grid, k,
+                                               RK_ALPHA[k], false, b->stream));
+                HIPCHK(hipEventRecord(evs[10 * s + 2 * k + 1], b->stream));
+            }
+            HIPCHK(jst::launch_post_mixed(b->d_cases, b->d_posts, b->nposts, 1, b->stream));
+        }
+        HIPCHK(hipStreamSynchronize(b->stream));
+        double sum = 0.0;
+        for (int q = 0; q < 5 * nsteps; ++q) {
+            float e = 0.f;
+            HIPCHK(hipEventElapsedTime(&e, evs[2 * q], evs[2 * q + 1]));
+            sum += e;
+        }
+        for (auto &e : evs) (void)hipEventDestroy(e);
+        *stage_ms = sum / (5.0 * nsteps);
+    }
+    for (int slot : touched)
+        if ((st = set_ctl_field(b, slot, &CaseCtl::timing, 0)) != JST_OK) return st;
+    return fetch_ctl(b);
+}
+
+jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant_out, int *grid_out)
+{
+    if (!b || trial_steps <= 0) return fail(JST_ERR_ARG, "bad arguments");
+    int best_v = b->variant, best_g = b->grid;
+    double best = 1e300;
+    for (int v = 0; v < MIX_NVARIANTS; ++v) {
+        // grid 0: one workgroup per tile; otherwise a persistent grid of k workgroups per CU
+        for (int g : {0, 256 * 2, 256 * 4}) {
+            jst_status st = jst_batch_set_tiling(b, v, g);
+            if (st != JST_OK) return st;
+            double ms = 0.0;
+            if ((st = jst_batch_time_steps(b, trial_steps, &ms, nullptr)) != JST_OK) return st;
+            if (ms < best) {
+                best = ms;
+                best_v = v;
+                best_g = g;
+            }
+        }
+    }
+    jst_status st = jst_batch_set_tiling(b, best_v, best_g);
+    if (st != JST_OK) return st;
+    if (variant_out) *variant_out = best_v;
+    if (grid_out) *grid_out = best_g;
+    return JST_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// Serving loop
+// ---------------------------------------------------------------------------------------
+jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
+                     jst_request_result *results, jst_serve_summary *summary)
+{
+    if (!reqs || n < 0 || !opts || !results) return fail(JST_ERR_ARG, "bad arguments");
+    jst_batch *b = nullptr;
+    jst_status st = jst_batch_create(JST_MIXED, opts->hbm_bytes, opts->max_cases, &b);
+    if (st != JST_OK) return st;
+    std::unique_ptr<jst_batch> guard(b);
+    const int spr = opts->steps_per_round > 0 ? opts->steps_per_round : 50;
+    std::vector<int> order(n);
+    for (int q = 0; q < n; ++q) {
+        order[q] = q;
+        results[q] = jst_request_result{-1.0, -1.0, 0, -1, NAN, NAN, NAN};
+    }
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int c) { return reqs[a].arrival_s < reqs[c].arrival_s; });
+    std::vector<int> slot_req(opts->max_cases, -1);
+    std::vector<long> last_step(opts->max_cases, 0);
+    double cell_stages = 0.0;
+    int next = 0, done = 0, nonphys = 0, maxc = 0, active = 0, tuned_for = -1;
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    while (done < n) {
+        // admission: in arrival order, while a slot and arena memory are free
+        bool admitted = false;
+        while (next < n && reqs[order[next]].arrival_s <= now()) {
+            const jst_request &r = reqs[order[next]];
+            int id = -1;
+            st = jst_batch_add(b, r.config_json, r.mesh_text, r.max_steps, &id);
+            if (st == JST_ERR_NOSPACE) break;
+            if (st != JST_OK) {
+                results[order[next]].status = -1;
+                ++done;
+                ++next;
+                continue;
+            }
+            results[order[next]].admit_s = now();
+            slot_req[id] = order[next];
+            last_step[id] = 0;
+            ++next;
+            ++active;
+            admitted = true;
+        }
+        maxc = std::max(maxc, active);
+        if (active == 0) {
+            if (next < n) {
+                const double wait = reqs[order[next]].arrival_s - now();
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
+            }
+            continue;
+        }
+        if (opts->tune && admitted && active != tuned_for && active >= 4) {
+            int v, g;
+            if ((st = jst_batch_tune(b, 3, &v, &g)) != JST_OK) return st;
+            tuned_for = active;
+        }
+        if ((st = jst_batch_step(b, spr, 1)) != JST_OK) return st;
+        // account and retire
+        for (int id = 0; id < opts->max_cases; ++id) {
+            const int rq = slot_req[id];
+            if (rq < 0) continue;
+            jst_case_info ci;
+            if ((st = jst_batch_info(b, id, &ci)) != JST_OK) return st;
+            cell_stages += (double)ci.cells * 5.0 * (double)(ci.step - last_step[id]);
+            last_step[id] = ci.step;
+            if (ci.status == JST_CASE_RUNNING) continue;
+            jst_request_result &res = results[rq];
+            res.done_s = now();
+            res.steps = ci.step;
+            res.status = ci.status;
+            res.residual = ci.residual;
+            if (ci.status != JST_CASE_NONPHYSICAL) {
+                if ((st = jst_batch_forces(b, id, &res.cl, &res.cd)) != JST_OK) return st;
+            } else {
+                ++nonphys;
+            }
+            if ((st = jst_batch_remove(b, id)) != JST_OK) return st;
+            slot_req[id] = -1;
+            --active;
+            ++done;
+        }
+    }
+    if (summary) {
+        std::vector<double> lat;
+        for (int q = 0; q < n; ++q)
+            if (results[q].done_s >= 0 && results[q].status >= 0)
+                lat.push_back(results[q].done_s - reqs[q].arrival_s);
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double p) -> double {
+            if (lat.empty()) return (double)NAN;
+            const double idx = p * (lat.size() - 1);
+            const size_t lo = (size_t)std::floor(idx), hi = (size_t)std::ceil(idx);
+            return lat[lo] + (lat[hi] - lat[lo]) * (idx - lo);
+        };
+        summary->wall_s = now();
+        summary->cell_stages = cell_stages;
+        summary->p50_s = pct(0.50);
+        summary->p99_s = pct(0.99);
+        summary->completed = (int)lat.size();
+        summary->nonphysical = nonphys;
+        summary->max_concurrent = maxc;
+    }
+    return JST_OK;
+}
+
+} // extern "C"

@@ -71,3 +71,46 @@ enum {
     DBG_FC = 0, DBG_FV = 5, DBG_FD = 10, DBG_S = 15, DBG_GRAD = 16, DBG_CHI = 24, DBG_FV1 = 25,
     DBG_NPLANES = 26
 };
+
+// ---------------------------------------------------------------------------------------
+// FP32-mixed path.  FP64: conserved state U^n / U^{n+1} (the RK accumulation), geometry
+// precompute, far-field Riemann state, residual and force reductions.  FP32: a shadow of
+// the perturbation dU = U - U_inf that the stage stencils read, and all stencil arithmetic
+// (fluxes in perturbation form so that the freestream part, which cancels over a closed
+// cell, never enters a single-precision sum).
+// ---------------------------------------------------------------------------------------
+struct MixConst {
+    // FP32 scheme constants
+    float gamma, gm1, r_gas, cfl_half, k2, k4;
+    float rho_inf, u_inf, v_inf, p_inf, t_inf, m_inf, n_inf, e_inf, h_inf, rnu_inf, nut_inf;
+    float suth_num, inv_t_ref, t_suth, inv_pr, inv_prt, cp, sigma, cv1_cubed;
+    float cb1, cb2, ct3, ct4, cw1, cw2, cw3_6, one_plus_cw3_6, fv3, inv_kappa2, rmax;
+    int lts;
+    // FP64 copies for the accumulation and the far-field state
+    double u64[5];                // U_inf (conserved)
+    PhysConst k;                  // full FP64 constants (far field, step control)
+};
+
+struct CaseMix {
+    int ni, nj, pitch;
+    int npost;                    // step-end reduction blocks of this case (fixed 1024-cell chunks)
+    double *u64[2];               // U^n / U^{n+1}, 5 planes each
+    float *s32[2];                // FP32 shadow dU, 5 planes each (stage ping-pong)
+    const float *tnx, *tny;       // FP32 tau normals (ni+1) x pitch
+    const float *nnx, *nny;       // FP32 n normals
+    const float *vol, *inv_vol, *inv_wd2;
+    const double *tnx64, *tny64, *tmx64, *tmy64, *cx64, *cy64; // far-field support (FP64)
+    float *ratio, *localdt;       // frozen per step
+    double *partial;              // per post-block partial sums
+    double *res_hist;
+    int res_cap;
+    CaseCtl *ctl;
+    float *dbg;                   // residual-terms export (may be null): 16 planes R0..R4,S,...
+    MixConst k;
+};
+
+// Post (step-end) work item: case, first cell (row-major), block index within the case.
+struct PostDesc {
+    int c, first, count, b;
+};
+#define JST_POST_CELLS 1024

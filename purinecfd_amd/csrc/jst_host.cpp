@@ -22,7 +22,9 @@
 #include "jst_dev.h"
 #include "jst_launch.h"
 
-namespace {
+#include "jst_common.h"
+
+namespace jsth {
 
 thread_local std::string g_err = "ok";
 
@@ -175,25 +177,7 @@ struct JParser {
 // ------------------------------------------------------------------------------------
 // Config (jst-rs config.rs)
 // ------------------------------------------------------------------------------------
-struct Config {
-    double gamma, r_gas, t_ref, t_suth, mu0;
-    double aoa, mach, cfl;
-    int halo;
-    double t_inf, p_inf;
-    long iteration = 10000;
-    double targetres = 1e-10;
-    double cv1, pr, prt, sigma, cb1, cb2, ct3, ct4, cw2, cw3, fv3, kappa, rmax;
-    double k2, k4;
-    // default-off extensions (absent from the reference; defaults reproduce it)
-    double nut_ratio = 0.1, reynolds = 0.0, ref_length = 1.0, residual_drop = 0.0;
-    bool lts = false;
-};
 
-struct Derived {
-    double cv, cp, cw1, cv1_cubed, c_inf, u_inf, v_inf, rho_inf, mu_inf, nut_inf;
-    bool supersonic;
-    double suth_num, inv_t_ref, inv_pr, inv_prt, inv_kappa2, cw3_6, one_plus_cw3_6, mu0_eff;
-};
 
 double powi_c(double x, int n)
 {
@@ -367,10 +351,6 @@ jst_status parse_config(const char *text, Config &c)
 // ------------------------------------------------------------------------------------
 // Mesh (jst-rs mesh.rs:109-189)
 // ------------------------------------------------------------------------------------
-struct Mesh {
-    int n_rings = 0, n_theta = 0;
-    std::vector<double> xy;
-};
 
 bool parse_usize(const std::string &s, long &out)
 {
@@ -524,7 +504,9 @@ std::string rust_exp(double v, int prec)
     return mant + "e" + std::to_string(ex);
 }
 
-} // namespace
+} // namespace jsth
+
+using namespace jsth;
 
 // ------------------------------------------------------------------------------------
 // Context
@@ -558,9 +540,12 @@ struct jst_ctx {
     int res_cap = 4096;
     CaseCtl *h_ctl = nullptr; // pinned mirror
     hipEvent_t ev[2] = {nullptr, nullptr};
+    jst_batch *mix = nullptr; // FP32-mixed: a one-case batch
+    int mix_id = -1;
 
     ~jst_ctx()
     {
+        if (mix) jst_batch_destroy(mix);
         if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : allocs) (void)hipFree(p);
         if (h_ctl) (void)hipHostFree(h_ctl);
@@ -580,8 +565,7 @@ struct jst_ctx {
     }
 };
 
-namespace {
-
+namespace jsth {
 PhysConst make_const(const Config &c, const Derived &d)
 {
     PhysConst k{};
@@ -622,6 +606,9 @@ PhysConst make_const(const Config &c, const Derived &d)
     k.lts = c.lts ? 1 : 0;
     return k;
 }
+} // namespace jsth
+
+namespace {
 
 // Tile list in an XCD-aware order: workgroup b is dispatched to XCD b % 8, so consecutive
 // row-major tiles (which share halo rows in L2) are dealt to the same XCD.
@@ -796,8 +783,20 @@ jst_status jst_create(const char *config_json, const char *mesh_text, jst_precis
                  x->cfg.halo);
         return fail(JST_ERR_CONFIG, buf);
     }
-    if (prec != JST_FP64) return fail(JST_ERR_ARG, "precision not available in this build");
     x->prec = prec;
+    if (prec == JST_MIXED) {
+        size_t bytes = 0;
+        jst_status sm = jst_case_footprint(config_json, mesh_text, prec, &bytes);
+        if (sm != JST_OK) return sm;
+        if ((sm = jst_batch_create(JST_MIXED, bytes + (4u << 20), 1, &x->mix)) != JST_OK) return sm;
+        if ((sm = jst_batch_add(x->mix, config_json, mesh_text, 0, &x->mix_id)) != JST_OK) return sm;
+        x->d = finish(x->cfg);
+        x->ni = x->mesh.n_rings - 1;
+        x->nj = x->mesh.n_theta;
+        *out = x.release();
+        return JST_OK;
+    }
+    if (prec != JST_FP64) return fail(JST_ERR_ARG, "unknown precision");
     x->d = finish(x->cfg);
     x->ni = x->mesh.n_rings - 1;
     x->nj = x->mesh.n_theta;
@@ -942,6 +941,14 @@ jst_status jst_dims(const jst_ctx *ctx, int *ni, int *nj)
 jst_status jst_progress(const jst_ctx *ctx, long *step, double *totaltime)
 {
     if (!ctx) return fail(JST_ERR_ARG, "ctx is NULL");
+    if (ctx->mix) {
+        jst_case_info ci;
+        jst_status st = jst_batch_info(ctx->mix, ctx->mix_id, &ci);
+        if (st != JST_OK) return st;
+        if (step) *step = ci.step;
+        if (totaltime) *totaltime = ci.totaltime;
+        return JST_OK;
+    }
     if (step) *step = (long)ctx->h_ctl->step;
     if (totaltime) *totaltime = ctx->h_ctl->totaltime;
     return JST_OK;
@@ -951,10 +958,11 @@ jst_status jst_run(jst_ctx *x, long max_steps, jst_step_cb cb, void *user, jst_r
                    jst_nonphysical *err)
 {
     if (!x) return fail(JST_ERR_ARG, "ctx is NULL");
+    const long limit = max_steps > 0 ? max_steps : x->cfg.iteration;
+    if (x->mix) return batch_run_case(x->mix, x->mix_id, limit, cb, user, out, false);
     jst_status st = ctl_fetch(x);
     if (st != JST_OK) return st;
     if (x->h_ctl->err_key != JST_ERR_NONE) return report_nonphysical(x, err);
-    const long limit = max_steps > 0 ? max_steps : x->cfg.iteration;
     const long long start = x->h_ctl->step;
     if ((st = ctl_set(x, &CaseCtl::stop_step, (long long)(start + limit))) != JST_OK) return st;
     if ((st = ctl_set(x, &CaseCtl::status, (int)JST_RUN_ACTIVE)) != JST_OK) return st;
@@ -999,6 +1007,12 @@ jst_status jst_advance(jst_ctx *x, int nsteps, double *res_out, jst_nonphysical 
         long n;
     } acc{res_out, 0};
     jst_report rep{};
+    auto acb = [](long, double r, void *u) {
+        Acc *a = (Acc *)u;
+        if (a->out) a->out[a->n] = r;
+        a->n++;
+    };
+    if (x->mix) return batch_run_case(x->mix, x->mix_id, nsteps, acb, &acc, &rep, true);
     // advance never stops on convergence: the reference's advance() has no stop criterion
     jst_status st = ctl_set(x, &CaseCtl::timing, 1);
     if (st != JST_OK) return st;
@@ -1017,6 +1031,13 @@ jst_status jst_advance(jst_ctx *x, int nsteps, double *res_out, jst_nonphysical 
 jst_status jst_residual(const jst_ctx *x, double *out)
 {
     if (!x || !out) return fail(JST_ERR_ARG, "NULL argument");
+    if (x->mix) {
+        jst_case_info ci;
+        jst_status st = jst_batch_info(x->mix, x->mix_id, &ci);
+        if (st != JST_OK) return st;
+        *out = ci.step > 0 ? ci.residual : NAN;
+        return JST_OK;
+    }
     *out = x->h_ctl->step > 0 ? x->h_ctl->residual : NAN;
     return JST_OK;
 }
@@ -1024,6 +1045,7 @@ jst_status jst_residual(const jst_ctx *x, double *out)
 jst_status jst_residual_terms(jst_ctx *x, double *dt_out)
 {
     if (!x) return fail(JST_ERR_ARG, "ctx is NULL");
+    if (x->mix) return jst_batch_residual_terms(x->mix, x->mix_id, dt_out);
     if (!x->dbg) {
         jst_status st = x->dalloc(x->dbg, (size_t)DBG_NPLANES * x->plane);
         if (st != JST_OK) return st;
@@ -1042,6 +1064,7 @@ jst_status jst_residual_terms(jst_ctx *x, double *dt_out)
 jst_status jst_get_field(jst_ctx *x, const char *name, double *dst, size_t n)
 {
     if (!x || !name || !dst) return fail(JST_ERR_ARG, "NULL argument");
+    if (x->mix) return jst_batch_get_field(x->mix, x->mix_id, name, dst, n);
     const int ni = x->ni, nj = x->nj, pitch = x->pitch;
     const size_t ncell = (size_t)ni * nj;
     std::string s(name);
@@ -1117,6 +1140,7 @@ jst_status jst_get_field(jst_ctx *x, const char *name, double *dst, size_t n)
 jst_status jst_set_state(jst_ctx *x, const double *u_planes, jst_nonphysical *err)
 {
     if (!x || !u_planes) return fail(JST_ERR_ARG, "NULL argument");
+    if (x->mix) return fail(JST_ERR_ARG, "set_state is available on the FP64 plain path");
     jst_status st = ctl_fetch(x);
     if (st != JST_OK) return st;
     const int ni = x->ni, nj = x->nj, pitch = x->pitch;
@@ -1158,6 +1182,16 @@ jst_status jst_set_state(jst_ctx *x, const double *u_planes, jst_nonphysical *er
 jst_status jst_forces(jst_ctx *x, double *fx, double *fy, double *cl, double *cd)
 {
     if (!x) return fail(JST_ERR_ARG, "ctx is NULL");
+    if (x->mix) {
+        double f4[4];
+        jst_status st = batch_forces(x->mix, x->mix_id, f4);
+        if (st != JST_OK) return st;
+        if (fx) *fx = f4[0];
+        if (fy) *fy = f4[1];
+        if (cl) *cl = f4[2];
+        if (cd) *cd = f4[3];
+        return JST_OK;
+    }
     double *dout = x->scratch + 2 * (size_t)x->nj;
     HIPCHK(jst::launch_forces_fp64(x->d_case, 0, x->scratch, dout, x->stream));
     double f[2];
@@ -1203,6 +1237,7 @@ jst_status jst_write_csv(jst_ctx *x, const char *path)
 jst_status jst_time_steps(jst_ctx *x, int nsteps, double *total_ms, double *stage_ms)
 {
     if (!x || nsteps <= 0) return fail(JST_ERR_ARG, "bad arguments");
+    if (x->mix) return jst_batch_time_steps(x->mix, nsteps, total_ms, stage_ms);
     jst_status st;
     if ((st = ctl_set(x, &CaseCtl::timing, 1)) != JST_OK) return st;
     if ((st = ctl_set(x, &CaseCtl::status, (int)JST_RUN_ACTIVE)) != JST_OK) return st;

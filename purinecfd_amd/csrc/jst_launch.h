@@ -27,4 +27,18 @@ hipError_t launch_post_fp64(const CaseDev *cases, const TileDesc *tiles, int nti
 hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, double *out,
                               hipStream_t s);
 
+// FP32-mixed stage kernel variants (tile rows x tile columns x threads); the autotuner picks
+// one per batch.  Results do not depend on the variant.
+struct MixVariant {
+    int ti, tj, nt;
+};
+#define MIX_NVARIANTS 3
+extern const MixVariant MIX_VARIANTS[MIX_NVARIANTS];
+hipError_t launch_stage_mixed(int variant, const CaseMix *cases, const TileDesc *tiles, int ntiles,
+                              int grid, int stage, double alpha, bool terms, hipStream_t s);
+hipError_t launch_post_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, int mode,
+                             hipStream_t s);
+hipError_t launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);
+hipError_t launch_shadow(const CaseMix *cases, int ci, size_t n5, hipStream_t s);
+
 } // namespace jst

@@ -31,7 +31,14 @@ MESH_ERRORS = {JST_ERR_IO: "Io", JST_ERR_BAD_HEADER: "BadHeader", JST_ERR_EMPTY:
 EXPORTS = ["jst_last_error", "jst_config_check", "jst_mesh_parse", "jst_create",
            "jst_create_from_paths", "jst_destroy", "jst_dims", "jst_advance", "jst_run",
            "jst_residual", "jst_residual_terms", "jst_get_field", "jst_set_state", "jst_forces",
-           "jst_write_csv", "jst_progress", "jst_time_steps"]
+           "jst_write_csv", "jst_progress", "jst_time_steps",
+           "jst_batch_create", "jst_batch_destroy", "jst_case_footprint", "jst_batch_add",
+           "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces",
+           "jst_batch_get_field", "jst_batch_residual_terms", "jst_batch_history",
+           "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_steps", "jst_batch_counts",
+           "jst_serve"]
+JST_ERR_NOSPACE = 11
+CASE_RUNNING, CASE_CONVERGED, CASE_STOPPED, CASE_NONPHYSICAL = 0, 1, 2, 3
 
 
 class JstError(RuntimeError):
@@ -69,6 +76,33 @@ class _Report(C.Structure):
                 ("converged", C.c_int)]
 
 
+class _CaseInfo(C.Structure):
+    _fields_ = [("step", C.c_long), ("status", C.c_int), ("residual", C.c_double),
+                ("res_first", C.c_double), ("totaltime", C.c_double), ("cells", C.c_long)]
+
+
+class _Request(C.Structure):
+    _fields_ = [("arrival_s", C.c_double), ("config_json", C.c_char_p),
+                ("mesh_text", C.c_char_p), ("max_steps", C.c_long)]
+
+
+class _RequestResult(C.Structure):
+    _fields_ = [("admit_s", C.c_double), ("done_s", C.c_double), ("steps", C.c_long),
+                ("status", C.c_int), ("residual", C.c_double), ("cl", C.c_double),
+                ("cd", C.c_double)]
+
+
+class _ServeOpts(C.Structure):
+    _fields_ = [("hbm_bytes", C.c_size_t), ("max_cases", C.c_int),
+                ("steps_per_round", C.c_int), ("tune", C.c_int)]
+
+
+class _ServeSummary(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("cell_stages", C.c_double), ("p50_s", C.c_double),
+                ("p99_s", C.c_double), ("completed", C.c_int), ("nonphysical", C.c_int),
+                ("max_concurrent", C.c_int)]
+
+
 STEP_CB = C.CFUNCTYPE(None, C.c_long, C.c_double, C.c_void_p)
 
 _lib = None
@@ -100,8 +134,27 @@ def lib():
         L.jst_write_csv.argtypes = [vp, C.c_char_p]
         L.jst_progress.argtypes = [vp, C.POINTER(C.c_long), dp]
         L.jst_time_steps.argtypes = [vp, ip, dp, dp]
+        sz = C.c_size_t
+        L.jst_batch_create.argtypes = [ip, sz, ip, C.POINTER(vp)]
+        L.jst_batch_destroy.argtypes = [vp]
+        L.jst_batch_destroy.restype = None
+        L.jst_case_footprint.argtypes = [C.c_char_p, C.c_char_p, ip, C.POINTER(sz)]
+        L.jst_batch_add.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_long, C.POINTER(ip)]
+        L.jst_batch_remove.argtypes = [vp, ip]
+        L.jst_batch_step.argtypes = [vp, ip, ip]
+        L.jst_batch_info.argtypes = [vp, ip, C.POINTER(_CaseInfo)]
+        L.jst_batch_forces.argtypes = [vp, ip, dp, dp]
+        L.jst_batch_get_field.argtypes = [vp, ip, C.c_char_p, dp, sz]
+        L.jst_batch_residual_terms.argtypes = [vp, ip, dp]
+        L.jst_batch_history.argtypes = [vp, ip, C.c_long, C.c_long, dp]
+        L.jst_batch_set_tiling.argtypes = [vp, ip, ip]
+        L.jst_batch_tune.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
+        L.jst_batch_time_steps.argtypes = [vp, ip, dp, dp]
+        L.jst_batch_counts.argtypes = [vp, C.POINTER(ip), C.POINTER(C.c_long)]
+        L.jst_serve.argtypes = [C.POINTER(_Request), ip, C.POINTER(_ServeOpts),
+                                C.POINTER(_RequestResult), C.POINTER(_ServeSummary)]
         for name in EXPORTS[1:]:
-            if name != "jst_destroy":
+            if name not in ("jst_destroy", "jst_batch_destroy"):
                 getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -301,3 +354,119 @@ class Solver:
         _check(lib().jst_time_steps(self._h, nsteps, C.byref(total),
                                     C.byref(stage) if per_stage else None))
         return total.value, (stage.value if per_stage else None)
+
+
+class Batch:
+    """Many cases resident in HBM, advanced together by one launch per RK stage (FP32-mixed).
+
+    No reference counterpart: the caller of the same loop for serving."""
+
+    def __init__(self, hbm_bytes: int = 8 << 30, max_cases: int = 256):
+        h = C.c_void_p()
+        _check(lib().jst_batch_create(MIXED, int(hbm_bytes), int(max_cases), C.byref(h)))
+        self._h = h
+        self._cells = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().jst_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @staticmethod
+    def footprint(config: str, mesh: str) -> int:
+        out = C.c_size_t()
+        _check(lib().jst_case_footprint(config.encode(), mesh.encode(), MIXED, C.byref(out)))
+        return out.value
+
+    def add(self, config: str, mesh: str, max_steps: int = 0) -> int:
+        cid = C.c_int()
+        _check(lib().jst_batch_add(self._h, config.encode(), mesh.encode(), int(max_steps),
+                                   C.byref(cid)))
+        return cid.value
+
+    def remove(self, cid: int) -> None:
+        _check(lib().jst_batch_remove(self._h, cid))
+
+    def step(self, nsteps: int, sync: bool = True) -> None:
+        _check(lib().jst_batch_step(self._h, int(nsteps), 1 if sync else 0))
+
+    def info(self, cid: int) -> dict:
+        ci = _CaseInfo()
+        _check(lib().jst_batch_info(self._h, cid, C.byref(ci)))
+        return {"step": ci.step, "status": ci.status, "residual": ci.residual,
+                "res_first": ci.res_first, "totaltime": ci.totaltime, "cells": ci.cells}
+
+    def run(self, max_rounds: int = 10**9, steps_per_round: int = 100) -> None:
+        """Step until no case is running."""
+        for _ in range(max_rounds):
+            self.step(steps_per_round)
+            n = C.c_int()
+            _check(lib().jst_batch_counts(self._h, C.byref(n), None))
+            if n.value == 0:
+                return
+
+    def forces(self, cid: int) -> dict:
+        cl, cd = C.c_double(), C.c_double()
+        _check(lib().jst_batch_forces(self._h, cid, C.byref(cl), C.byref(cd)))
+        return {"cl": cl.value, "cd": cd.value}
+
+    def get(self, cid: int, name: str, ni: int, nj: int) -> np.ndarray:
+        rows = ni + 1 if name.startswith("tau_") else ni
+        buf = np.zeros(rows * nj)
+        _check(lib().jst_batch_get_field(self._h, cid, name.encode(), _dp(buf), buf.size))
+        return buf
+
+    def residual_terms(self, cid: int) -> float:
+        dt = C.c_double()
+        _check(lib().jst_batch_residual_terms(self._h, cid, C.byref(dt)))
+        return dt.value
+
+    def history(self, cid: int, first: int, n: int) -> np.ndarray:
+        out = np.zeros(n)
+        _check(lib().jst_batch_history(self._h, cid, int(first), int(n), _dp(out)))
+        return out
+
+    def set_tiling(self, variant: int, grid: int = 0) -> None:
+        _check(lib().jst_batch_set_tiling(self._h, int(variant), int(grid)))
+
+    def tune(self, trial_steps: int = 5):
+        v, g = C.c_int(), C.c_int()
+        _check(lib().jst_batch_tune(self._h, int(trial_steps), C.byref(v), C.byref(g)))
+        return v.value, g.value
+
+    def time_steps(self, nsteps: int, per_stage: bool = False):
+        total, stage = C.c_double(), C.c_double()
+        _check(lib().jst_batch_time_steps(self._h, int(nsteps), C.byref(total),
+                                          C.byref(stage) if per_stage else None))
+        return total.value, (stage.value if per_stage else None)
+
+    def counts(self):
+        n, cells = C.c_int(), C.c_long()
+        _check(lib().jst_batch_counts(self._h, C.byref(n), C.byref(cells)))
+        return n.value, cells.value
+
+
+def serve(requests, hbm_bytes: int = 16 << 30, max_cases: int = 64, steps_per_round: int = 50,
+          tune: bool = False):
+    """Serving loop over ``requests`` = [(arrival_s, config, mesh, max_steps), ...].
+
+    Returns (per-request results, summary) from jst_serve."""
+    n = len(requests)
+    keep = []
+    reqs = (_Request * max(n, 1))()
+    for q, (t, cfg, mesh, steps) in enumerate(requests):
+        cb, mb = cfg.encode(), mesh.encode()
+        keep += [cb, mb]
+        reqs[q] = _Request(float(t), cb, mb, int(steps))
+    res = (_RequestResult * max(n, 1))()
+    summ = _ServeSummary()
+    opts = _ServeOpts(int(hbm_bytes), int(max_cases), int(steps_per_round), 1 if tune else 0)
+    _check(lib().jst_serve(reqs, n, C.byref(opts), res, C.byref(summ)))
+    out = [{"admit_s": r.admit_s, "done_s": r.done_s, "steps": r.steps, "status": r.status,
+            "residual": r.residual, "cl": r.cl, "cd": r.cd} for r in res[:n]]
+    return out, {"wall_s": summ.wall_s, "cell_stages": summ.cell_stages, "p50_s": summ.p50_s,
+                 "p99_s": summ.p99_s, "completed": summ.completed,
+                 "nonphysical": summ.nonphysical, "max_concurrent": summ.max_concurrent}

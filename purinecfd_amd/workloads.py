@@ -76,10 +76,15 @@ def c3_naca_large() -> dict:
 C4_SIZES = [(129, 65), (193, 97), (257, 129), (385, 193), (513, 257)]
 
 
-def c4_batch(n: int = 64, seed: int = 20261203) -> list[dict]:
+def c4_batch(n: int = 64, seed: int = 20261203, first_spacing: float = 1e-3) -> list[dict]:
     """Uneven batch from a seed: NACA 4-digit t ~ U[8%,18%], m ~ U[0,4%] (p = 0.4),
     O-meshes drawn from C4_SIZES (closed rings), M ~ U[0.3,0.85], AoA ~ U[-4,8] deg,
-    Re ~ logU[1e5,1e7], first spacing 1e-5 c, local time stepping, SA nu~/nu = 3."""
+    Re ~ logU[1e5,1e7], SA nu~/nu = 3, CFL 1.0 with the reference's global time step.
+
+    First wall spacing 1e-3 chord: the reference scheme sizes its time step from the
+    convective spectral radius only, so finer wall spacing at the low-Re end of this
+    distribution violates the explicit viscous limit and diverges at the first steps (checked
+    with the CPU oracle at 1e-5 and 1e-4 chord)."""
     rng = random.Random(seed)
     out = []
     for k in range(n):
@@ -89,14 +94,44 @@ def c4_batch(n: int = 64, seed: int = 20261203) -> list[dict]:
         mach = rng.uniform(0.3, 0.85)
         aoa = rng.uniform(-4.0, 8.0)
         re = 10.0 ** rng.uniform(5.0, 7.0)
-        mesh, q = meshgen.naca_mesh(pts - 1, rings, m, 0.4, t, 1e-5, 30.0, closed=True)
-        cfg = make_config(mach=mach, aoa=aoa, cfl=2.0, reynolds=re, nut_ratio=3.0, lts=True,
+        mesh, q = meshgen.naca_mesh(pts - 1, rings, m, 0.4, t, first_spacing, 30.0, closed=True)
+        cfg = make_config(mach=mach, aoa=aoa, cfl=1.0, reynolds=re, nut_ratio=3.0, lts=False,
                           iteration=20000, residual_drop=6.0)
         out.append({"name": f"c4_{k:02d}_naca{m*100:.1f}-{t*100:.1f}_{pts}x{rings}",
                     "config": cfg, "mesh": mesh, "stretch": q, "ni": rings - 1, "nj": pts - 1,
-                    "mach": mach, "aoa": aoa, "re": re})
+                    "mach": mach, "aoa": aoa, "re": re, "m": m, "t": t})
     return out
 
 
 def golden_config_text() -> str:
     return (Path(__file__).resolve().parent.parent / "tests" / "golden" / "config.json").read_text()
+
+
+CYLINDER_SIZES = [(40, 17), (80, 33), (160, 65)]
+
+
+def c5_requests(seconds: float = 60.0, rate: float = 20.0, seed: int = 20261204,
+                cylinder_fraction: float = 0.5) -> list[dict]:
+    """Serving stream: Poisson arrivals (mean `rate` solves/s) over `seconds`; each request is
+    a cylinder case (40x17, 80x33 or 160x65 points, M 0.2, Re 1e3, CFL 2, converged at a
+    5-decade residual drop, at most 4000 steps) with probability `cylinder_fraction`, else a
+    draw from the C4 distribution (at most 2000 steps, 6-decade drop)."""
+    rng = random.Random(seed)
+    pool = c4_batch(n=64, seed=seed)
+    out, t = [], 0.0
+    while True:
+        t += rng.expovariate(rate)
+        if t > seconds:
+            break
+        if rng.random() < cylinder_fraction:
+            pts, rings = CYLINDER_SIZES[rng.randrange(len(CYLINDER_SIZES))]
+            mesh, _ = meshgen.cylinder_mesh(pts, rings, 0.5, 20.0)
+            cfg = make_config(mach=0.2, aoa=0.0, cfl=2.0, reynolds=1e3, nut_ratio=3.0,
+                              iteration=4000, residual_drop=5.0)
+            out.append({"arrival_s": t, "config": cfg, "mesh": mesh, "max_steps": 4000,
+                        "name": f"cyl_{pts}x{rings}"})
+        else:
+            w = pool[rng.randrange(len(pool))]
+            out.append({"arrival_s": t, "config": w["config"], "mesh": w["mesh"],
+                        "max_steps": 2000, "name": w["name"]})
+    return out

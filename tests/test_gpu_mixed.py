@@ -1,0 +1,196 @@
+"""FP32-mixed path, batching and serving on the GPU.
+
+Tolerances (stated per north_star): against the FP64 oracle, |dCL|, |dCD| <= 1e-4 per case,
+field L2 relative error reported and bounded; within the mixed path itself, results are
+bitwise identical alone vs in a mixed batch and for every tile variant (each face and cell is
+evaluated from the case's own data only, reductions have a fixed per-case tree).
+"""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+from conftest import config_dict, config_text, ellipse_mesh, mesh_text
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def jst():
+    from purinecfd_amd import jst as mod
+    mod.lib()
+    return mod
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_mixed_residual_matches_oracle_fp32_level(jst, oracle_lib):
+    """One residual evaluation (R = Fc - Fv - Fd per component, and S) of the mixed path
+    after 3 steps of the golden 17x40 case, against the oracle: field-normed <= 2e-4."""
+    text = mesh_text("cyl_17x40.txt")
+    s = jst.Solver(config_text(), text, jst.MIXED)
+    o = oracle_lib.OracleSolver(config_dict(), text)
+    s.run(3)
+    o.run(3)
+    dt = s.residual_terms()
+    odt = o.timestep()
+    o.residual_terms()
+    assert abs(dt - odt) <= 1e-5 * odt
+    worst = 0.0
+    for k in range(5):
+        want = o.get(f"Fc{k}") - o.get(f"Fv{k}") - o.get(f"Fd{k}")
+        got = s.get(f"R{k}")
+        err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+        worst = max(worst, err)
+        assert err <= 2e-4, (k, err)
+    want = o.get("S")
+    err = np.max(np.abs(s.get("S") - want)) / np.max(np.abs(want))
+    assert err <= 2e-4, ("S", err)
+    print(f"mixed residual worst field-normed error vs oracle: {max(worst, err):.2e}")
+
+
+def test_mixed_cylinder_c1_cl_cd_within_1e4(jst, oracle_lib):
+    """BASELINE cylinder case, 2000 iterations: mixed vs FP64 oracle |dCL|,|dCD| <= 1e-4."""
+    from purinecfd_amd import workloads
+    w = workloads.c1_cylinder()
+    s = jst.Solver(w["config"], w["mesh"], jst.MIXED)
+    o = oracle_lib.OracleSolver(json.loads(w["config"]), w["mesh"])
+    rep = s.run(2000)
+    orep = o.run(2000)
+    assert rep.steps == orep["steps"] == 2000
+    f, of = s.forces(), o.forces()
+    dcl, dcd = abs(f["cl"] - of["cl"]), abs(f["cd"] - of["cd"])
+    l2 = {n: rel_l2(s.get(n), o.get(n)) for n in ("rho", "p", "u", "v", "nut")}
+    print(f"C1 mixed vs oracle: dCL {dcl:.2e} dCD {dcd:.2e} (CD {of['cd']:.6f}); L2 {l2}")
+    assert dcl <= 1e-4 and dcd <= 1e-4
+    assert l2["rho"] <= 1e-5 and l2["p"] <= 1e-5 and l2["u"] <= 1e-4
+
+
+def _cases():
+    cfg_a = config_dict()
+    cfg_b = config_dict()
+    cfg_b["simulation"].update({"AOA": 3.0, "Ma": 0.3, "CFL": 1.0})
+    cfg_c = config_dict()
+    cfg_c["simulation"].update({"AOA": -2.0, "Ma": 0.5})
+    return [
+        (json.dumps(cfg_a), mesh_text("cyl_17x40.txt")),
+        (json.dumps(cfg_b), ellipse_mesh(33, 96, 1.08)),
+        (json.dumps(cfg_c), ellipse_mesh(21, 64, 1.15)),
+    ]
+
+
+def _fields(b, cid, text):
+    from purinecfd_amd.jst import Mesh
+    mesh = Mesh.parse(text)
+    ni, nj = mesh.ni(), mesh.nj()
+    return {n: b.get(cid, n, ni, nj) for n in ("rho", "u", "v", "p", "nut")}
+
+
+def test_batch_alone_equals_in_the_mix_bitwise(jst):
+    cases = _cases()
+    alone = []
+    for cfg, text in cases:
+        b = jst.Batch(1 << 30, 4)
+        cid = b.add(cfg, text, 60)
+        b.run(steps_per_round=25)
+        alone.append((_fields(b, cid, text), b.forces(cid), b.info(cid)))
+        b.close()
+    b = jst.Batch(1 << 30, 8)
+    ids = [b.add(cfg, text, 60) for cfg, text in cases]
+    b.run(steps_per_round=25)
+    for (cfg, text), cid, (fa, fo, ia) in zip(cases, ids, alone):
+        fm = _fields(b, cid, text)
+        for n in fa:
+            assert np.array_equal(fa[n], fm[n]), n
+        assert b.forces(cid) == fo
+        assert b.info(cid)["step"] == ia["step"] == 60
+
+
+def test_tile_variant_and_grid_do_not_change_results(jst):
+    cfg, text = _cases()[1]
+    ref = None
+    for variant, grid in ((0, 0), (1, 0), (2, 0), (0, 512), (2, 1024)):
+        b = jst.Batch(1 << 30, 2)
+        b.set_tiling(variant, grid)
+        cid = b.add(cfg, text, 40)
+        b.run(steps_per_round=20)
+        f = _fields(b, cid, text)
+        if ref is None:
+            ref = f
+        else:
+            for n in f:
+                assert np.array_equal(f[n], ref[n]), (variant, grid, n)
+        b.close()
+
+
+def test_graph_replays_match_eager_launches(jst):
+    """A HIP-graph replayed step sequence checked replay by replay against the per-step
+    residual history of a fresh batch stepped one synchronised step at a time."""
+    cfg, text = _cases()[2]
+    b1 = jst.Batch(1 << 30, 2)
+    c1 = b1.add(cfg, text, 30)
+    for _ in range(30):
+        b1.step(1)
+    h1 = b1.history(c1, 1, 30)
+    b2 = jst.Batch(1 << 30, 2)
+    c2 = b2.add(cfg, text, 30)
+    for _ in range(3):
+        b2.step(10)
+    h2 = b2.history(c2, 1, 30)
+    assert np.array_equal(h1, h2)
+
+
+def test_diverging_case_is_evicted_others_unaffected(jst):
+    cases = _cases()[:2]
+    bad = config_dict()
+    bad["simulation"]["CFL"] = 200.0
+    b = jst.Batch(1 << 30, 4)
+    ids = [b.add(cfg, text, 50) for cfg, text in cases]
+    bid = b.add(json.dumps(bad), ellipse_mesh(13, 40), 50)
+    b.run(steps_per_round=10)
+    assert b.info(bid)["status"] == jst.CASE_NONPHYSICAL
+    b_ok = jst.Batch(1 << 30, 4)
+    ids_ok = [b_ok.add(cfg, text, 50) for cfg, text in cases]
+    b_ok.run(steps_per_round=10)
+    for (cfg, text), a, c in zip(cases, ids, ids_ok):
+        assert b.info(a)["status"] == jst.CASE_STOPPED
+        fa, fc = _fields(b, a, text), _fields(b_ok, c, text)
+        for n in fa:
+            assert np.array_equal(fa[n], fc[n]), n
+
+
+def test_serving_loop_results_match_alone(jst):
+    """Requests arriving while others run are admitted into the running batch; each result
+    equals the same case run alone (bitwise), and latency percentiles are reported."""
+    cases = _cases()
+    reqs = [(0.0, cases[0][0], cases[0][1], 80), (0.05, cases[1][0], cases[1][1], 60),
+            (0.1, cases[2][0], cases[2][1], 70), (0.12, cases[0][0], cases[0][1], 40)]
+    res, summ = jst.serve(reqs, hbm_bytes=1 << 30, max_cases=3, steps_per_round=10)
+    assert summ["completed"] == 4 and summ["nonphysical"] == 0
+    assert summ["p99_s"] >= summ["p50_s"] > 0
+    for (t, cfg, text, steps), r in zip(reqs, res):
+        b = jst.Batch(1 << 30, 1)
+        cid = b.add(cfg, text, steps)
+        b.run(steps_per_round=10)
+        f = b.forces(cid)
+        assert r["steps"] == steps and r["status"] == jst.CASE_STOPPED
+        assert r["cl"] == f["cl"] and r["cd"] == f["cd"]
+        b.close()
+
+
+def test_admission_by_footprint(jst):
+    """A batch whose arena holds one case admits the second only after the first retires."""
+    cfg, text = _cases()[1]
+    fp = jst.Batch.footprint(cfg, text)
+    b = jst.Batch(int(fp * 1.5), 4)
+    a = b.add(cfg, text, 10)
+    with pytest.raises(jst.JstError) as e:
+        b.add(cfg, text, 10)
+    assert e.value.status == jst.JST_ERR_NOSPACE
+    b.run(steps_per_round=10)
+    b.remove(a)
+    b.add(cfg, text, 10)
