@@ -202,6 +202,9 @@ jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant, int *grid
  * one stage launch from a second pass with events around every stage launch. */
 jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, double *stage_ms);
 jst_status jst_batch_counts(const jst_batch *b, int *active_cases, long *active_cells);
+/* Profiling calibration (no reference counterpart): one device-to-device copy of `bytes`
+ * bytes, so that profiler byte counters can be scaled against a transfer of known size. */
+jst_status jst_copy_probe(size_t bytes, double *ms);
 
 /* One solve request of the serving loop. */
 typedef struct {

@@ -1,7 +1,7 @@
 // jst_batch.cpp — batched FP32-mixed driver and the serving loop (include/jst_gpu.h).
 //
 // A batch keeps every admitted case resident in one HBM arena and advances all running cases
-// with one launch per RK stage (plus one step-end launch): the stage kernel walks a tile list
+// with one launch per RK stage plus a small step-end launch: the stage kernel walks a tile list
 // spanning every case, so small and large meshes share the GPU.  One step of the whole batch
 // is captured once into a HIP graph and replayed; the case table, tile list and per-case
 // control blocks live in device memory, so the graph stays valid while cases advance,
@@ -36,6 +36,7 @@ using namespace jsth;
 namespace {
 
 const double RK_ALPHA[5] = {0.25, 1.0 / 6.0, 0.375, 0.5, 1.0}; // jst-rs config.rs:295
+const int GRAPH_STEPS = 16;
 
 inline size_t r256(size_t n) { return (n + 255) & ~(size_t)255; }
 
@@ -78,7 +79,7 @@ struct Arena {
 
 // Byte layout of one case inside its arena block.
 struct CaseLayout {
-    size_t u64[2], s32[2], g64[14], g32[7], ratio, localdt, partial, res_hist, posts, scratch, dbg;
+    size_t u64[2], s32[2], g64[14], g32[7], ratio[2], localdt[2], rpart, dpart, res_hist, posts, scratch, dbg;
     size_t total;
 };
 
@@ -101,9 +102,10 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     // FP32 geometry: tnx tny (fplane) | nnx nny vol inv_vol inv_wd2 (plane)
     for (int q = 0; q < 2; ++q) L.g32[q] = carve(fplane * 4);
     for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
-    L.ratio = carve(plane * 4);
-    L.localdt = carve(plane * 4);
-    L.partial = carve((size_t)npost * 8);
+    for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
+    for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
+    L.rpart = carve((size_t)ni * (pitch / 32) * 8);
+    L.dpart = carve((size_t)ni * (pitch / 32) * 8);
     L.res_hist = carve((size_t)res_cap * 8);
     L.posts = carve((size_t)npost * sizeof(PostDesc));
     L.scratch = carve((2 * (size_t)nj + 2) * 8);
@@ -191,13 +193,14 @@ struct jst_batch {
     TileDesc *d_tiles = nullptr;
     size_t tiles_cap = 0;
     int ntiles = 0;
-    PostDesc *d_posts = nullptr;
-    size_t posts_cap = 0;
-    int nposts = 0;
+    int *d_slots = nullptr; // slots of the batch's cases (one step-end block each)
+    int nslots = 0;
     int variant = 0, grid = 0;
     bool dirty = true;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;      // one step
     hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph_k = nullptr;    // GRAPH_STEPS steps (host launch cost amortised)
+    hipGraphExec_t gexec_k = nullptr;
     bool use_graph = true;
     int res_cap = 4096;
     double *d_scratch_out = nullptr;
@@ -209,8 +212,10 @@ struct jst_batch {
         if (stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (gexec_k) (void)hipGraphExecDestroy(gexec_k);
+        if (graph_k) (void)hipGraphDestroy(graph_k);
         for (void *p : {(void *)arena.base, (void *)d_cases, (void *)d_ctl, (void *)d_tiles,
-                        (void *)d_posts, (void *)d_scratch_out, (void *)d_view})
+                        (void *)d_slots, (void *)d_scratch_out, (void *)d_view})
             if (p) (void)hipFree(p);
         if (h_ctl) (void)hipHostFree(h_ctl);
         for (auto e : ev)
@@ -225,24 +230,28 @@ jst_status drop_graph(jst_batch *b)
 {
     if (b->gexec) HIPCHK(hipGraphExecDestroy(b->gexec));
     if (b->graph) HIPCHK(hipGraphDestroy(b->graph));
+    if (b->gexec_k) HIPCHK(hipGraphExecDestroy(b->gexec_k));
+    if (b->graph_k) HIPCHK(hipGraphDestroy(b->graph_k));
     b->gexec = nullptr;
     b->graph = nullptr;
+    b->gexec_k = nullptr;
+    b->graph_k = nullptr;
     return JST_OK;
 }
 
 // Rebuild the tile list (XCD-aware: consecutive tiles of a case are dealt to the same XCD,
-// since workgroup w runs on XCD w % 8) and the step-end work list.
+// since workgroup w runs on XCD w % 8) and the step-end slot list.
 jst_status rebuild_lists(jst_batch *b)
 {
     const jst::MixVariant v = jst::MIX_VARIANTS[b->variant];
     std::vector<TileDesc> nat;
-    std::vector<PostDesc> posts;
+    std::vector<int> slots;
     for (auto &hc : b->cases) {
         if (!hc) continue;
         const int nti = (hc->ni + v.ti - 1) / v.ti, ntj = (hc->nj + v.tj - 1) / v.tj;
         for (int a = 0; a < nti; ++a)
             for (int c = 0; c < ntj; ++c) nat.push_back(TileDesc{hc->slot, a * v.ti, c * v.tj, a * ntj + c});
-        posts.insert(posts.end(), hc->posts.begin(), hc->posts.end());
+        slots.push_back(hc->slot);
     }
     const int n = (int)nat.size(), per = (n + 7) / 8;
     std::vector<TileDesc> tiles;
@@ -257,59 +266,80 @@ jst_status rebuild_lists(jst_batch *b)
         b->tiles_cap = std::max<size_t>(tiles.size() * 2, 1024);
         HIPCHK(hipMalloc(&b->d_tiles, b->tiles_cap * sizeof(TileDesc)));
     }
-    if (posts.size() > b->posts_cap) {
-        if (b->d_posts) HIPCHK(hipFree(b->d_posts));
-        b->posts_cap = std::max<size_t>(posts.size() * 2, 1024);
-        HIPCHK(hipMalloc(&b->d_posts, b->posts_cap * sizeof(PostDesc)));
-    }
     if (!tiles.empty())
         HIPCHK(hipMemcpyAsync(b->d_tiles, tiles.data(), tiles.size() * sizeof(TileDesc),
                               hipMemcpyHostToDevice, b->stream));
-    if (!posts.empty())
-        HIPCHK(hipMemcpyAsync(b->d_posts, posts.data(), posts.size() * sizeof(PostDesc),
+    if (!slots.empty())
+        HIPCHK(hipMemcpyAsync(b->d_slots, slots.data(), slots.size() * sizeof(int),
                               hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     b->ntiles = (int)tiles.size();
-    b->nposts = (int)posts.size();
+    b->nslots = (int)slots.size();
     jst_status st = drop_graph(b);
     if (st != JST_OK) return st;
     b->dirty = false;
     return JST_OK;
 }
 
+// One RK step of every running case: five stage launches (the fifth also writes the step-end
+// partials) and the one-block-per-case step end.
 jst_status launch_one_step(jst_batch *b)
 {
     for (int k = 0; k < 5; ++k)
         HIPCHK(jst::launch_stage_mixed(b->variant, b->d_cases, b->d_tiles, b->ntiles, b->grid, k,
                                        RK_ALPHA[k], false, b->stream));
-    HIPCHK(jst::launch_post_mixed(b->d_cases, b->d_posts, b->nposts, 1, b->stream));
+    HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
     return JST_OK;
 }
 
+// Capture `steps` consecutive steps of the whole batch into one executable graph.
+jst_status capture_steps(jst_batch *b, int steps, hipGraph_t *g, hipGraphExec_t *x)
+{
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    jst_status st = JST_OK;
+    for (int q = 0; q < steps && st == JST_OK; ++q) st = launch_one_step(b);
+    hipGraph_t gg = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &gg);
+    if (st != JST_OK) return st;
+    HIPCHK(e);
+    *g = gg;
+    HIPCHK(hipGraphInstantiate(x, gg, nullptr, nullptr, 0));
+    return JST_OK;
+}
+
+// Capture (without running) the graphs that queueing nsteps steps will use.
+jst_status prepare_graphs(jst_batch *b, int nsteps)
+{
+    jst_status st;
+    if (!b->use_graph) return JST_OK;
+    if (nsteps >= GRAPH_STEPS && !b->gexec_k &&
+        (st = capture_steps(b, GRAPH_STEPS, &b->graph_k, &b->gexec_k)) != JST_OK)
+        return st;
+    if (nsteps % GRAPH_STEPS && !b->gexec && (st = capture_steps(b, 1, &b->graph, &b->gexec)) != JST_OK)
+        return st;
+    return JST_OK;
+}
+
+// Queue nsteps steps: graphs of GRAPH_STEPS steps, then single-step graphs for the rest.
+// Steps past a case's stop are no-ops on the device, so queueing never overshoots a case.
 jst_status launch_steps(jst_batch *b, int nsteps)
 {
     if (b->dirty) {
         jst_status st = rebuild_lists(b);
         if (st != JST_OK) return st;
     }
-    if (b->ntiles == 0) return JST_OK;
-    if (b->use_graph && !b->gexec) {
-        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        jst_status st = launch_one_step(b);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(b->stream, &g);
-        if (st != JST_OK) return st;
-        HIPCHK(e);
-        b->graph = g;
-        HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
-    }
-    for (int s = 0; s < nsteps; ++s) {
-        if (b->gexec) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
-        else {
+    if (b->ntiles == 0 || nsteps <= 0) return JST_OK;
+    if (!b->use_graph) {
+        for (int s = 0; s < nsteps; ++s) {
             jst_status st = launch_one_step(b);
             if (st != JST_OK) return st;
         }
+        return JST_OK;
     }
+    jst_status st = prepare_graphs(b, nsteps);
+    if (st != JST_OK) return st;
+    for (int s = 0; s < nsteps / GRAPH_STEPS; ++s) HIPCHK(hipGraphLaunch(b->gexec_k, b->stream));
+    for (int s = 0; s < nsteps % GRAPH_STEPS; ++s) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
     return JST_OK;
 }
 
@@ -430,6 +460,7 @@ jst_status batch_run_case(jst_batch *b, int id, long limit, jst_step_cb cb, void
             return fail(JST_ERR_NONPHYSICAL, buf);
         }
         if (c.status != JST_RUN_ACTIVE) break;
+        if (nd <= 0) return fail(JST_ERR_HIP, "the device completed no step of an active case");
         batch = std::min(batch * 2, std::min(512, b->res_cap));
     }
     if ((st = set_ctl_field(b, id, &CaseCtl::timing, 0)) != JST_OK) return st;
@@ -484,6 +515,7 @@ jst_status jst_batch_create(jst_precision prec, size_t hbm_bytes, int max_cases,
     b->arena.free_[0] = hbm_bytes;
     HIPCHK(hipMalloc((void **)&b->d_cases, sizeof(CaseMix) * (size_t)max_cases));
     HIPCHK(hipMalloc((void **)&b->d_ctl, sizeof(CaseCtl) * (size_t)max_cases));
+    HIPCHK(hipMalloc((void **)&b->d_slots, sizeof(int) * (size_t)max_cases));
     HIPCHK(hipMalloc((void **)&b->d_scratch_out, 64));
     HIPCHK(hipMalloc((void **)&b->d_view, sizeof(CaseDev)));
     HIPCHK(hipHostMalloc((void **)&b->h_ctl, sizeof(CaseCtl) * (size_t)max_cases, hipHostMallocDefault));
@@ -541,6 +573,7 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     cm.nj = hc->nj;
     cm.pitch = hc->pitch;
     cm.npost = hc->npost;
+    cm.nchunk = hc->pitch / 32;
     for (int q = 0; q < 2; ++q) {
         cm.u64[q] = D(L.u64[q]);
         cm.s32[q] = F(L.s32[q]);
@@ -564,9 +597,12 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     cm.tmy64 = g.tmy;
     cm.cx64 = g.cx;
     cm.cy64 = g.cy;
-    cm.ratio = F(L.ratio);
-    cm.localdt = F(L.localdt);
-    cm.partial = D(L.partial);
+    for (int q = 0; q < 2; ++q) {
+        cm.ratio[q] = F(L.ratio[q]);
+        cm.localdt[q] = F(L.localdt[q]);
+    }
+    cm.rpart = D(L.rpart);
+    cm.dpart = D(L.dpart);
     cm.res_hist = D(L.res_hist);
     cm.res_cap = b->res_cap;
     cm.ctl = b->d_ctl + slot;
@@ -613,7 +649,7 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     HIPCHK(hipMemcpyAsync(b->d_ctl + slot, &b->h_ctl[slot], sizeof(CaseCtl), hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(b->d_cases + slot, &cm, sizeof(CaseMix), hipMemcpyHostToDevice, s));
     HIPCHK(jst::launch_shadow(b->d_cases, slot, 5 * hc->plane, s));
-    HIPCHK(jst::launch_post_mixed(b->d_cases, hc->d_posts, hc->npost, 0, s));
+    HIPCHK(jst::launch_dt_mixed(b->d_cases, hc->d_posts, hc->npost, s));
     HIPCHK(hipStreamSynchronize(s));
     // the node buffer is only needed for the geometry precompute
     b->arena.release(hc->off + L.total, xy_bytes);
@@ -707,7 +743,7 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out)
     TileDesc *dt = nullptr;
     HIPCHK(hipMalloc(&dt, t.size() * sizeof(TileDesc)));
     HIPCHK(hipMemcpyAsync(dt, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice, b->stream));
-    HIPCHK(jst::launch_post_mixed(b->d_cases, hc->d_posts, hc->npost, 0, b->stream));
+    HIPCHK(jst::launch_dt_mixed(b->d_cases, hc->d_posts, hc->npost, b->stream));
     HIPCHK(jst::launch_stage_mixed(b->variant, b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true,
                                    b->stream));
     jst_status st = fetch_ctl(b);
@@ -757,8 +793,8 @@ jst_status jst_batch_get_field(jst_batch *b, int case_id, const char *name, doub
     if (s.size() == 2 && s[0] == 'R' && s[1] >= '0' && s[1] <= '4')
         return plane32(hc->dev.dbg + (size_t)(s[1] - '0') * hc->plane);
     if (s == "S") return plane32(hc->dev.dbg + 5 * hc->plane);
-    if (s == "localdt") return plane32(hc->dev.localdt);
-    if (s == "ratio") return plane32(hc->dev.ratio);
+    if (s == "localdt") return plane32(hc->dev.localdt[b->h_ctl[case_id].rot]);
+    if (s == "ratio") return plane32(hc->dev.ratio[b->h_ctl[case_id].rot]);
     std::vector<double> h(5 * hc->plane);
     if ((st = download(b, hc->dev.u64[b->h_ctl[case_id].rot], h.size() * 8, h.data())) != JST_OK)
         return st;
@@ -813,7 +849,8 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
             touched.push_back(hc->slot);
         }
     if (b->dirty && (st = rebuild_lists(b)) != JST_OK) return st;
-    // build (and warm) the graph outside the timed region
+    // build (and warm) the graphs outside the timed region
+    if ((st = prepare_graphs(b, nsteps)) != JST_OK) return st;
     if ((st = launch_steps(b, 1)) != JST_OK) return st;
     HIPCHK(hipEventRecord(b->ev[0], b->stream));
     if ((st = launch_steps(b, nsteps)) != JST_OK) return st;
@@ -832,7 +869,7 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
                                                RK_ALPHA[k], false, b->stream));
                 HIPCHK(hipEventRecord(evs[10 * s + 2 * k + 1], b->stream));
             }
-            HIPCHK(jst::launch_post_mixed(b->d_cases, b->d_posts, b->nposts, 1, b->stream));
+            HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
         }
         HIPCHK(hipStreamSynchronize(b->stream));
         double sum = 0.0;
@@ -846,7 +883,41 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
     }
     for (int slot : touched)
         if ((st = set_ctl_field(b, slot, &CaseCtl::timing, 0)) != JST_OK) return st;
-    return fetch_ctl(b);
+    if ((st = fetch_ctl(b)) != JST_OK) return st;
+    // a diverged case stops its own kernels early: a timing that includes one is not valid
+    for (int slot : touched)
+        if (b->h_ctl[slot].err_key != JST_ERR_NONE) {
+            const long long cell = (long long)(b->h_ctl[slot].err_key & 0xFFFFFFFFull);
+            char buf[160];
+            snprintf(buf, sizeof buf, "non-physical state in case %d at cell (%lld, %lld) during timing",
+                     slot, cell / b->cases[slot]->nj, cell % b->cases[slot]->nj);
+            return fail(JST_ERR_NONPHYSICAL, buf);
+        }
+    return JST_OK;
+}
+
+// Calibration probe for the HBM byte counters: one device-to-device copy of `bytes`.
+jst_status jst_copy_probe(size_t bytes, double *ms)
+{
+    void *a = nullptr, *c = nullptr;
+    hipEvent_t e0, e1;
+    HIPCHK(hipMalloc(&a, bytes));
+    HIPCHK(hipMalloc(&c, bytes));
+    HIPCHK(hipMemset(a, 1, bytes));
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, nullptr));
+    HIPCHK(hipMemcpyAsync(c, a, bytes, hipMemcpyDeviceToDevice, nullptr));
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (ms) *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipFree(a));
+    HIPCHK(hipFree(c));
+    return JST_OK;
 }
 
 jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant_out, int *grid_out)
@@ -937,11 +1008,13 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
         }
         if ((st = jst_batch_step(b, spr, 1)) != JST_OK) return st;
         // account and retire
+        bool progressed = false;
         for (int id = 0; id < opts->max_cases; ++id) {
             const int rq = slot_req[id];
             if (rq < 0) continue;
             jst_case_info ci;
             if ((st = jst_batch_info(b, id, &ci)) != JST_OK) return st;
+            progressed = progressed || ci.step > last_step[id] || ci.status != JST_CASE_RUNNING;
             cell_stages += (double)ci.cells * 5.0 * (double)(ci.step - last_step[id]);
             last_step[id] = ci.step;
             if (ci.status == JST_CASE_RUNNING) continue;
@@ -960,6 +1033,7 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
             --active;
             ++done;
         }
+        if (!progressed) return fail(JST_ERR_HIP, "the device completed no step of an active case");
     }
     if (summary) {
         std::vector<double> lat;

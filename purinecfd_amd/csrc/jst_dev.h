@@ -93,15 +93,17 @@ struct MixConst {
 
 struct CaseMix {
     int ni, nj, pitch;
-    int npost;                    // step-end reduction blocks of this case (fixed 1024-cell chunks)
+    int npost;                    // dt-refresh blocks of this case (fixed 1024-cell chunks)
+    int nchunk;                   // 32-column chunks per row (pitch / 32)
     double *u64[2];               // U^n / U^{n+1}, 5 planes each
     float *s32[2];                // FP32 shadow dU, 5 planes each (stage ping-pong)
     const float *tnx, *tny;       // FP32 tau normals (ni+1) x pitch
     const float *nnx, *nny;       // FP32 n normals
     const float *vol, *inv_vol, *inv_wd2;
     const double *tnx64, *tny64, *tmx64, *tmy64, *cx64, *cy64; // far-field support (FP64)
-    float *ratio, *localdt;       // frozen per step
-    double *partial;              // per post-block partial sums
+    float *ratio[2], *localdt[2]; // frozen per step; [rot] is read by the step, [rot^1] written
+    double *rpart;                // residual partial sums per (row, 32-column chunk)
+    double *dpart;                // next-dt minima per (row, 32-column chunk)
     double *res_hist;
     int res_cap;
     CaseCtl *ctl;
@@ -109,7 +111,7 @@ struct CaseMix {
     MixConst k;
 };
 
-// Post (step-end) work item: case, first cell (row-major), block index within the case.
+// dt-refresh work item: case, first cell (row-major), block index within the case.
 struct PostDesc {
     int c, first, count, b;
 };

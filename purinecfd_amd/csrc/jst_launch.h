@@ -32,12 +32,12 @@ hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, dou
 struct MixVariant {
     int ti, tj, nt;
 };
-#define MIX_NVARIANTS 3
+#define MIX_NVARIANTS 4
 extern const MixVariant MIX_VARIANTS[MIX_NVARIANTS];
 hipError_t launch_stage_mixed(int variant, const CaseMix *cases, const TileDesc *tiles, int ntiles,
                               int grid, int stage, double alpha, bool terms, hipStream_t s);
-hipError_t launch_post_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, int mode,
-                             hipStream_t s);
+hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s);
+hipError_t launch_dt_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, hipStream_t s);
 hipError_t launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);
 hipError_t launch_shadow(const CaseMix *cases, int ci, size_t n5, hipStream_t s);
 

@@ -23,7 +23,8 @@ __device__ __forceinline__ int wrapj(int j, int nj)
     j %= nj;
     return j < 0 ? j + nj : j;
 }
-__device__ __forceinline__ size_t cidx(int i, int j, int pitch) { return (size_t)i * pitch + j; }
+// 32-bit cell offsets: the host rejects cases whose planes exceed 2^31 / 8 elements
+__device__ __forceinline__ int cidx(int i, int j, int pitch) { return i * pitch + j; }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
@@ -46,14 +47,14 @@ __device__ __forceinline__ void prim_from_du(const MixConst &k, const float d[5]
 
 // jst-rs boundary.rs:112-185 in FP64 from the FP64 reconstruction U_inf + dU of the three
 // outermost cells.  Out: rho p t vx vy e nut.
-__device__ void far_state64(const CaseMix &C, const float *S, int jw, double s[7])
+__device__ __noinline__ void far_state64(const CaseMix &C, const float *S, int jw, double s[7])
 {
     const PhysConst &k = C.k.k;
     const int ni = C.ni, pitch = C.pitch;
-    const size_t plane = (size_t)ni * pitch;
+    const int plane = ni * pitch;
     double vx[3], vy[3], tt[3], nu[3];
     for (int m = 0; m < 3; ++m) {
-        const size_t c = cidx(ni - 1 - m, jw, pitch);
+        const int c = cidx(ni - 1 - m, jw, pitch);
         double u[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) u[q] = C.k.u64[q] + (double)S[q * plane + c];
@@ -64,7 +65,7 @@ __device__ void far_state64(const CaseMix &C, const float *S, int jw, double s[7
         tt[m] = p / (k.r_gas * u[0]);
         nu[m] = u[4] * inv;
     }
-    const size_t f = cidx(ni, jw, pitch);
+    const int f = cidx(ni, jw, pitch);
     const double nx = C.tnx64[f], ny = C.tny64[f], fmx = C.tmx64[f], fmy = C.tmy64[f];
     const double len = hypot(nx, ny);
     const double gamma = k.gamma;
@@ -106,7 +107,7 @@ __device__ void far_state64(const CaseMix &C, const float *S, int jw, double s[7
     } else {
         double dist[3];
         for (int m = 0; m < 3; ++m) {
-            const size_t c = cidx(ni - 1 - m, jw, pitch);
+            const int c = cidx(ni - 1 - m, jw, pitch);
             dist[m] = hypot(C.cx64[c] - fmx, C.cy64[c] - fmy);
         }
         const double l1 = dist[0], l2 = dist[1], l3 = dist[2];
@@ -236,12 +237,17 @@ struct MixSmem {
     float ratio[H1][W1];
     float d[8][H1][W1];
     float src[TI][TJ];
+    int col[W3]; // wrapped column index of R3 column c (j0 - 3 + c mod NJ)
 };
 
 } // namespace
 
 // ---------------------------------------------------------------------------------------
-// Fused FP32 stage, grid-strided over a tile list spanning any number of cases.
+// Fused FP32 stage, grid-strided over a tile list spanning any number of cases.  Stage 5
+// also computes the step-end quantities of its own cells: the next local dt and V/dt (into
+// the other half of their double buffer) and, per row and 32-column chunk, the residual
+// partial sum and the dt minimum.  Partials are keyed by cell position, not by tile, so the
+// step end does not depend on the tile shape or on the other cases of the batch.
 // ---------------------------------------------------------------------------------------
 template <int TI, int TJ, int NT, bool TERMS>
 __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict__ cases,
@@ -252,8 +258,10 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
     constexpr int H3 = S::H3, W3 = S::W3, H1 = S::H1, W1 = S::W1;
     constexpr int NFT = (TI + 1) * TJ, NFN = TI * (TJ + 1);
     constexpr int FPT = (NFT + NFN + NT - 1) / NT;
+    static_assert(TJ % 32 == 0 && (TI * TJ) % NT == 0 && NT % 64 == 0, "tile shape");
     __shared__ S sm;
     const int tid = threadIdx.x;
+    const bool last_stage = !TERMS && stage == 4;
 
     for (int tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
         const TileDesc tl = tiles[tix];
@@ -261,7 +269,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
         const MixConst &k = C.k;
         CaseCtl *ctl = C.ctl;
         const int ni = C.ni, nj = C.nj, pitch = C.pitch;
-        const size_t plane = (size_t)ni * pitch;
+        const int plane = ni * pitch;
         const int i0 = tl.i0, j0 = tl.j0;
         const int status = *(volatile int *)&ctl->status;
         const long long step = *(volatile long long *)&ctl->step;
@@ -270,40 +278,46 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
         const long long seq = step * 5 + stage;
         if (!TERMS) {
             if (status != JST_RUN_ACTIVE) continue; // uniform per block
-            if (ek != JST_ERR_NONE && (long long)(ek >> 32) < seq) continue;
+            if (ek != JST_ERR_NONE && (long long)(ek >> 32) < seq) continue; // failed earlier
         }
         const float *src = TERMS ? C.s32[rot] : C.s32[rot ^ (stage & 1)];
         float *sdst = C.s32[rot ^ ((stage & 1) ^ 1)];
         const double *former = C.u64[rot];
         double *u_next = C.u64[rot ^ 1];
+        const float *ratio_now = C.ratio[rot];
         const double dt = ctl->dt;
+        const double inv_cfl = 1.0 / k.k.cfl;
+
+        for (int c = tid; c < W3; c += NT) {
+            int jw = j0 - JST_H + c;
+            while (jw < 0) jw += nj;
+            while (jw >= nj) jw -= nj;
+            sm.col[c] = jw;
+        }
+        __syncthreads();
 
         // ---- phase 1: R3 load + ghost synthesis (perturbation variables) ----
         for (int idx = tid; idx < H3 * W3; idx += NT) {
             const int r = idx / W3, c = idx - r * W3;
-            const int i = i0 - JST_H + r, jj = j0 - JST_H + c;
-            const int jw = wrapj(jj, nj);
+            const int i = i0 - JST_H + r; // >= -3
             float d[5], dp, du, dv, dT, nut;
-            if (i >= 0 && i < ni) {
-                const size_t q = cidx(i, jw, pitch);
+            if (i < ni) {
+                // interior row i, or row -1-i for the wall mirror (boundary.rs:30-51)
+                const int q = cidx(i >= 0 ? i : -1 - i, sm.col[c], pitch);
 #pragma unroll
                 for (int m = 0; m < 5; ++m) d[m] = src[m * plane + q];
                 prim_from_du(k, d, dp, du, dv, dT, nut);
-            } else if (i < 0 && i >= -JST_H) {
-                // wall mirror of row -1-i: scalars copied, u, v, nu~ negated (boundary.rs:30-51)
-                const size_t q = cidx(-1 - i, jw, pitch);
-#pragma unroll
-                for (int m = 0; m < 5; ++m) d[m] = src[m * plane + q];
-                prim_from_du(k, d, dp, du, dv, dT, nut);
-                d[1] = -d[1] - 2.0f * k.m_inf;
-                d[2] = -d[2] - 2.0f * k.n_inf;
-                d[4] = -d[4] - 2.0f * k.rnu_inf;
-                du = -du - 2.0f * k.u_inf;
-                dv = -dv - 2.0f * k.v_inf;
-                nut = -nut;
-            } else if (i >= ni && i < ni + JST_H) {
+                if (i < 0) { // scalars copied, u, v, nu~ negated
+                    d[1] = -d[1] - 2.0f * k.m_inf;
+                    d[2] = -d[2] - 2.0f * k.n_inf;
+                    d[4] = -d[4] - 2.0f * k.rnu_inf;
+                    du = -du - 2.0f * k.u_inf;
+                    dv = -dv - 2.0f * k.v_inf;
+                    nut = -nut;
+                }
+            } else if (i < ni + JST_H) {
                 double s[7];
-                far_state64(C, src, jw, s);
+                far_state64(C, src, sm.col[c], s);
                 const double *U = C.k.u64;
                 d[0] = (float)(s[0] - U[0]);
                 d[1] = (float)(s[0] * s[3] - U[1]);
@@ -333,7 +347,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
             const int r = idx / W1, c = idx - r * W1;
             int i = i0 - 1 + r;
             i = i < 0 ? 0 : (i > ni - 1 ? ni - 1 : i);
-            sm.ratio[r][c] = C.ratio[cidx(i, wrapj(j0 - 1 + c, nj), pitch)];
+            sm.ratio[r][c] = ratio_now[cidx(i, sm.col[c + 2], pitch)];
         }
         __syncthreads();
 
@@ -351,7 +365,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
             const int r = idx / W1, c = idx - r * W1;
             const int i = i0 - 1 + r, jj = j0 - 1 + c;
             const int R = r + 2, Q = c + 2;
-            if (i < -1 || i > ni) continue;
+            if (i > ni) continue;
             float g[8];
             if (i == ni) {
 #pragma unroll
@@ -359,13 +373,12 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
             } else {
                 const int ie = i < 0 ? 0 : i;
                 const int Re = i < 0 ? R + 1 : R;
-                const int jw = wrapj(jj, nj);
-                const int jp = jw + 1 < nj ? jw + 1 : 0;
-                const float upx = C.tnx[cidx(ie + 1, jw, pitch)], upy = C.tny[cidx(ie + 1, jw, pitch)];
-                const float dnx = C.tnx[cidx(ie, jw, pitch)], dny = C.tny[cidx(ie, jw, pitch)];
-                const float lfx = C.nnx[cidx(ie, jw, pitch)], lfy = C.nny[cidx(ie, jw, pitch)];
-                const float rtx = C.nnx[cidx(ie, jp, pitch)], rty = C.nny[cidx(ie, jp, pitch)];
-                const float inv_v = C.inv_vol[cidx(ie, jw, pitch)];
+                const int q0 = cidx(ie, sm.col[Q], pitch), qp = cidx(ie, sm.col[Q + 1], pitch);
+                const float upx = C.tnx[q0 + pitch], upy = C.tny[q0 + pitch];
+                const float dnx = C.tnx[q0], dny = C.tny[q0];
+                const float lfx = C.nnx[q0], lfy = C.nny[q0];
+                const float rtx = C.nnx[qp], rty = C.nny[qp];
+                const float inv_v = C.inv_vol[q0];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) {
                     const float(*fld)[W3] = f == 0 ? sm.a.du : (f == 1 ? sm.a.dv : (f == 2 ? sm.a.dT : sm.a.nu));
@@ -388,7 +401,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
 #pragma unroll
             for (int m = 0; m < 8; ++m) sm.d[m][r][c] = d[m];
             if (r >= 1 && r <= TI && c >= 1 && c <= TJ && i < ni && jj < nj) {
-                const size_t q = cidx(i, jj, pitch);
+                const int q = cidx(i, jj, pitch);
                 sm.src[r - 1][c - 1] = sa_source32(k, sm.a.u[0][R][Q], sm.a.u[4][R][Q], chi, fv1, g,
                                                    C.inv_wd2[q], C.vol[q]);
             }
@@ -403,7 +416,7 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
             const int idx = tid + q * NT;
             gsel[q] = -1;
             if (idx >= NFT + NFN) continue;
-            float a[5], b[5], fc[5], lam, e2, e4, nx, ny, dA[8], dB[8];
+            float lam, e2, e4, nx, ny, dA[8], dB[8];
             float um2[5], um1[5], u0[5], up1[5];
             float s0, s1, s2, s3;
             if (idx < NFT) { // tau face (f, j): cells f-1 (below) and f
@@ -430,14 +443,13 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
                     dA[m] = sm.d[m][rf][c + 1];
                     dB[m] = sm.d[m][rf + 1][c + 1];
                 }
-                gsel[q] = idx;
             } else { // n face (i, g): cells g-1 (left) and g
                 const int id2 = idx - NFT;
                 const int r = id2 / (TJ + 1), cg = id2 - r * (TJ + 1);
                 const int i = i0 + r, gcol = j0 + cg;
                 if (i >= ni || gcol > nj) continue;
                 const int R = r + 3, Q = cg + 3;
-                const int gw = gcol < nj ? gcol : gcol - nj;
+                const int gw = sm.col[Q];
                 nx = C.nnx[cidx(i, gw, pitch)];
                 ny = C.nny[cidx(i, gw, pitch)];
 #pragma unroll
@@ -457,20 +469,15 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
                     dA[m] = sm.d[m][r + 1][cg];
                     dB[m] = sm.d[m][r + 1][cg + 1];
                 }
-                gsel[q] = idx;
             }
+            gsel[q] = idx;
             lam *= k.cfl_half;
             const float mx = fmaxf(fmaxf(fmaxf(s0, s1), s2), s3);
             e2 = k.k2 * mx;
             e4 = fmaxf(k.k4 - e2, 0.0f);
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                a[m] = um1[m];
-                b[m] = u0[m];
-            }
-            euler_flux_pert(k, a, b, nx, ny, fc);
+            float fc[5];
+            euler_flux_pert(k, um1, u0, nx, ny, fc);
             const float hx = 0.5f * nx, hy = 0.5f * ny;
-            G[q][0] = fc[0];
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
                 const float d1 = u0[m] - um1[m];
@@ -499,98 +506,168 @@ __global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict
         __syncthreads();
 
         // ---- phase 4: residual and FP64 update of the tile's own cells ----
-        for (int o = tid; o < TI * TJ; o += NT) {
+        for (int o = tid; o < TI * TJ; o += NT) { // uniform trip count (TI*TJ % NT == 0)
             const int ti = o / TJ, tj = o - ti * TJ;
             const int i = i0 + ti, j = j0 + tj;
-            if (i >= ni || j >= nj) continue;
-            float R5[5];
+            const bool valid = i < ni && j < nj;
+            double d2 = 0.0, dmin = __builtin_huge_val();
+            if (valid) {
+                float R5[5];
 #pragma unroll
-            for (int m = 0; m < 5; ++m)
-                R5[m] = sm.f.gt[m][ti + 1][tj] - sm.f.gt[m][ti][tj] + sm.f.gn[m][ti][tj + 1] - sm.f.gn[m][ti][tj];
-            const size_t q = cidx(i, j, pitch);
-            if (TERMS) { // export R = Fc - Fv - Fd per component and S separately
+                for (int m = 0; m < 5; ++m)
+                    R5[m] = sm.f.gt[m][ti + 1][tj] - sm.f.gt[m][ti][tj] + sm.f.gn[m][ti][tj + 1] - sm.f.gn[m][ti][tj];
+                const int q = cidx(i, j, pitch);
+                if (TERMS) { // export R = Fc - Fv - Fd per component and S separately
 #pragma unroll
-                for (int m = 0; m < 5; ++m) C.dbg[m * plane + q] = R5[m];
-                C.dbg[5 * plane + q] = sm.src[ti][tj];
-                continue;
-            }
-            R5[4] -= sm.src[ti][tj];
-            const double a_dt = k.lts ? alpha * (double)C.localdt[q] : alpha * dt;
-            const double sc = a_dt * (double)C.inv_vol[q];
-            double un[5];
+                    for (int m = 0; m < 5; ++m) C.dbg[m * plane + q] = R5[m];
+                    C.dbg[5 * plane + q] = sm.src[ti][tj];
+                } else {
+                    R5[4] -= sm.src[ti][tj];
+                    const double a_dt = k.lts ? alpha * (double)C.localdt[rot][q] : alpha * dt;
+                    const double sc = a_dt * (double)C.inv_vol[q];
+                    double un[5];
 #pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                un[m] = former[m * plane + q] - (double)R5[m] * sc;
-                sdst[m * plane + q] = (float)(un[m] - k.u64[m]);
-            }
-            if (stage == 4) {
+                    for (int m = 0; m < 5; ++m) {
+                        un[m] = former[m * plane + q] - (double)R5[m] * sc;
+                        sdst[m * plane + q] = (float)(un[m] - k.u64[m]);
+                    }
+                    // NonPhysical test (state.rs:100-117) without a division: rho > 0 there,
+                    // so p <= 1e-15 <=> p * rho <= 1e-15 * rho
+                    const double prho = (k.k.gamma - 1.0) * (un[3] * un[0] - 0.5 * (un[1] * un[1] + un[2] * un[2]));
+                    if (isnan(un[0]) || un[0] <= 1e-15 || isnan(prho) || prho <= 1e-15 * un[0])
+                        atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
+                                                     (unsigned long long)((long long)i * nj + j));
+                    if (last_stage) {
+                        const double inv = 1.0 / un[0];
+                        const double vx = un[1] * inv, vy = un[2] * inv;
+                        const double p = prho * inv;
 #pragma unroll
-                for (int m = 0; m < 5; ++m) u_next[m * plane + q] = un[m];
+                        for (int m = 0; m < 5; ++m) u_next[m * plane + q] = un[m];
+                        const double dr = un[0] - former[q];
+                        d2 = dr * dr;
+                        // next step's local dt and V/dt (jst-rs timestep.rs:24-47)
+                        const int jp = j + 1 < nj ? j + 1 : 0;
+                        const int qu = cidx(i + 1, j, pitch), qr = cidx(i, jp, pitch);
+                        const double aa = 0.5 * ((double)C.tnx[q] + (double)C.tnx[qu]);
+                        const double bb = 0.5 * ((double)C.tny[q] + (double)C.tny[qu]);
+                        const double cc = 0.5 * ((double)C.nnx[q] + (double)C.nnx[qr]);
+                        const double dd = 0.5 * ((double)C.nny[q] + (double)C.nny[qr]);
+                        const double cs = sqrt(k.k.gamma * fmax(p, 0.0) * inv);
+                        const double conv = fabs(vx * aa + vy * bb) + fabs(vx * cc + vy * dd);
+                        const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(cc * cc + dd * dd));
+                        const double vol = (double)C.vol[q];
+                        const double ldt = k.k.cfl * vol / (conv + acou);
+                        C.localdt[rot ^ 1][q] = (float)ldt;
+                        C.ratio[rot ^ 1][q] = (float)((conv + acou) * inv_cfl); // V / dt_local
+                        dmin = ldt;
+                    }
+                }
             }
-            bool bad = isnan(un[0]) || un[0] <= 1e-15;
-            if (!bad) {
-                const double inv = 1.0 / un[0];
-                const double vx = un[1] * inv, vy = un[2] * inv;
-                const double p = (k.k.gamma - 1.0) * (un[3] - un[0] * (vx * vx + vy * vy) * 0.5);
-                bad = isnan(p) || p <= 1e-15;
+            if (last_stage) {
+                // this row's 32-column chunk: fixed 32-lane butterflies (sum, min)
+                for (int off = 16; off > 0; off >>= 1) {
+                    d2 += __shfl_xor(d2, off, 32);
+                    dmin = fmin(dmin, __shfl_xor(dmin, off, 32));
+                }
+                if ((tid & 31) == 0 && i < ni && j < nj) {
+                    const int ch = i * C.nchunk + (j >> 5);
+                    C.rpart[ch] = d2;
+                    C.dpart[ch] = dmin;
+                }
             }
-            if (bad)
-                atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
-                                             (unsigned long long)((long long)i * nj + j));
         }
         __syncthreads(); // LDS is reused by the next tile of this block
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// Step end (mode 1) / dt refresh (mode 0).  Work items are fixed 1024-cell row-major chunks
-// of each case, so the reduction tree depends neither on the stage tile shape nor on the
-// batch composition.
+// dt refresh (case creation, residual-terms export): local dt and V/dt of the current state
+// into buffer [rot], global minimum into ctl->dt.  Work items are fixed 1024-cell chunks.
+// ---------------------------------------------------------------------------------------
+// ---------------------------------------------------------------------------------------
+// End of step, one block per case: residual sqrt(sum (rho^{n+1} - rho^n)^2 / NI / NJ)
+// (jst-rs solver.rs:190-200) and the next global dt from the per-(row, 32-column) partials in
+// a fixed order, totaltime += dt, step += 1, buffer rotation, and the stop decision
+// (convergence / relative drop / step limit, or the non-physical record).
 // ---------------------------------------------------------------------------------------
 template <int NT>
-__global__ __launch_bounds__(NT) void mix_post_kernel(const CaseMix *__restrict__ cases,
-                                                      const PostDesc *__restrict__ posts, int mode)
+__global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__ cases,
+                                                     const int *__restrict__ slots)
 {
-    __shared__ double red_s[NT];
+    static_assert(NT == 256, "fixed reduction tree");
+    __shared__ double red_s[NT / 64], red_m[NT / 64];
+    const CaseMix &C = cases[slots[blockIdx.x]];
+    CaseCtl *ctl = C.ctl;
+    const int tid = threadIdx.x;
+    if (*(volatile int *)&ctl->status != JST_RUN_ACTIVE) return;
+    const int n = C.ni * C.nchunk;
+    double s = 0.0, m = __builtin_huge_val();
+    for (int t = tid; t < n; t += NT) {
+        s += C.rpart[t];
+        m = fmin(m, C.dpart[t]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        m = fmin(m, __shfl_xor(m, off, 64));
+    }
+    if ((tid & 63) == 0) {
+        red_s[tid >> 6] = s;
+        red_m[tid >> 6] = m;
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    CaseCtl &v = *ctl;
+    const PhysConst &k = C.k.k;
+    if (v.err_key != JST_ERR_NONE) {
+        v.status = JST_RUN_NONPHYSICAL;
+        return;
+    }
+    const double tot = (red_s[0] + red_s[1]) + (red_s[2] + red_s[3]);
+    const double res = sqrt(tot / (double)C.ni / (double)C.nj);
+    const int rot = v.rot;
+    v.totaltime = v.totaltime + v.dt;
+    const long long st = v.step + 1;
+    v.step = st;
+    v.rot = rot ^ 1;
+    v.residual = res;
+    C.res_hist[(st - 1) % C.res_cap] = res;
+    if (st == 1) v.res_first = res;
+    bool conv = res < k.targetres;
+    if (k.residual_drop > 0.0 && v.res_first > 0.0)
+        conv = conv || res < v.res_first * pow(10.0, -k.residual_drop);
+    if (!v.timing) {
+        if (conv) v.status = JST_RUN_CONVERGED;
+        else if (st >= v.stop_step) v.status = JST_RUN_STOPPED;
+    }
+    v.dt = fmin(fmin(red_m[0], red_m[1]), fmin(red_m[2], red_m[3]));
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void mix_dt_kernel(const CaseMix *__restrict__ cases,
+                                                    const PostDesc *__restrict__ posts)
+{
     __shared__ double red_m[NT];
     __shared__ int last;
     const PostDesc pd = posts[blockIdx.x];
     const CaseMix &C = cases[pd.c];
     const PhysConst &k = C.k.k;
     CaseCtl *ctl = C.ctl;
-    const int ni = C.ni, nj = C.nj, pitch = C.pitch;
-    const size_t plane = (size_t)ni * pitch;
+    const int nj = C.nj, pitch = C.pitch;
+    const int plane = C.ni * pitch;
     const int tid = threadIdx.x;
-    const int status = *(volatile int *)&ctl->status;
-    const unsigned long long ek = *(volatile unsigned long long *)&ctl->err_key;
     const int rot = *(volatile int *)&ctl->rot;
-    if (mode == 1) {
-        if (status != JST_RUN_ACTIVE) return;
-        if (ek != JST_ERR_NONE) {
-            if (tid == 0) ctl->status = JST_RUN_NONPHYSICAL;
-            return;
-        }
-    }
-    const double *unew = mode == 1 ? C.u64[rot ^ 1] : C.u64[rot];
-    const double *uold = C.u64[rot];
-    double acc = 0.0, dmin = __builtin_huge_val();
+    const double *u = C.u64[rot];
+    double dmin = __builtin_huge_val();
     for (int o = tid; o < pd.count; o += NT) {
         const int cell = pd.first + o;
         const int i = cell / nj, j = cell - i * nj;
-        const size_t c = cidx(i, j, pitch);
-        double u[5];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) u[m] = unew[m * plane + c];
-        if (mode == 1) {
-            const double d = u[0] - uold[c];
-            acc += d * d;
-        }
-        const double inv = 1.0 / u[0];
-        const double vx = u[1] * inv, vy = u[2] * inv;
-        const double p = (k.gamma - 1.0) * (u[3] - u[0] * (vx * vx + vy * vy) * 0.5);
+        const int c = cidx(i, j, pitch);
+        const double rho = u[c], inv = 1.0 / rho;
+        const double vx = u[plane + c] * inv, vy = u[2 * plane + c] * inv;
+        const double p = (k.gamma - 1.0) * (u[3 * plane + c] - rho * (vx * vx + vy * vy) * 0.5);
         const double cs = sqrt(k.gamma * p * inv);
         const int jp = j + 1 < nj ? j + 1 : 0;
-        const size_t cu = cidx(i + 1, j, pitch), cr = cidx(i, jp, pitch);
+        const int cu = cidx(i + 1, j, pitch), cr = cidx(i, jp, pitch);
         const double aa = 0.5 * ((double)C.tnx[c] + (double)C.tnx[cu]);
         const double bb = 0.5 * ((double)C.tny[c] + (double)C.tny[cu]);
         const double ccv = 0.5 * ((double)C.nnx[c] + (double)C.nnx[cr]);
@@ -599,63 +676,30 @@ __global__ __launch_bounds__(NT) void mix_post_kernel(const CaseMix *__restrict_
         const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(ccv * ccv + ddv * ddv));
         const double vol = (double)C.vol[c];
         const double ldt = k.cfl * vol / (conv + acou);
-        C.localdt[c] = (float)ldt;
-        C.ratio[c] = (float)(vol / ldt);
+        C.localdt[rot][c] = (float)ldt;
+        C.ratio[rot][c] = (float)(vol / ldt);
         dmin = fmin(dmin, ldt);
     }
-    red_s[tid] = acc;
     red_m[tid] = dmin;
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            red_s[tid] = red_s[tid] + red_s[tid + s];
-            red_m[tid] = fmin(red_m[tid], red_m[tid + s]);
-        }
+        if (tid < s) red_m[tid] = fmin(red_m[tid], red_m[tid + s]);
         __syncthreads();
     }
     if (tid == 0) {
-        C.partial[pd.b] = red_s[0];
         const double m = red_m[0];
         if (m == m) atomicMin(&ctl->dt_acc, (unsigned long long)__double_as_longlong(m));
         __threadfence();
         last = atomicAdd(&ctl->blocks_done, 1u) == (unsigned int)(C.npost - 1);
     }
     __syncthreads();
-    if (!last) return;
+    if (!last || tid != 0) return;
     __threadfence();
-    double s = 0.0;
-    for (int t = tid; t < C.npost; t += NT) s += ((volatile double *)C.partial)[t];
-    red_s[tid] = s;
-    __syncthreads();
-    for (int h = NT / 2; h > 0; h >>= 1) {
-        if (tid < h) red_s[tid] = red_s[tid] + red_s[tid + h];
-        __syncthreads();
-    }
-    if (tid == 0) {
-        volatile CaseCtl *v = ctl;
-        const double new_dt = __longlong_as_double((long long)v->dt_acc);
-        if (mode == 1) {
-            const double res = sqrt(red_s[0] / (double)ni / (double)nj);
-            v->totaltime = v->totaltime + v->dt;
-            const long long st = v->step + 1;
-            v->step = st;
-            v->rot = rot ^ 1;
-            v->residual = res;
-            C.res_hist[(st - 1) % C.res_cap] = res;
-            if (st == 1) v->res_first = res;
-            bool conv = res < k.targetres;
-            if (k.residual_drop > 0.0 && v->res_first > 0.0)
-                conv = conv || res < v->res_first * pow(10.0, -k.residual_drop);
-            if (!v->timing) {
-                if (conv) v->status = JST_RUN_CONVERGED;
-                else if (st >= v->stop_step) v->status = JST_RUN_STOPPED;
-            }
-        }
-        v->dt = new_dt;
-        v->dt_acc = JST_INF_BITS;
-        v->blocks_done = 0u;
-        __threadfence();
-    }
+    volatile CaseCtl *v = ctl;
+    v->dt = __longlong_as_double((long long)v->dt_acc);
+    v->dt_acc = JST_INF_BITS;
+    v->blocks_done = 0u;
+    __threadfence();
 }
 
 // FP64 geometry -> FP32 copies, and the FP32 shadow of the current FP64 state.
@@ -693,7 +737,8 @@ static hipError_t stage_launch(const CaseMix *cases, const TileDesc *tiles, int 
     return hipGetLastError();
 }
 
-const MixVariant MIX_VARIANTS[MIX_NVARIANTS] = {{16, 32, 256}, {8, 64, 256}, {8, 32, 256}};
+const MixVariant MIX_VARIANTS[MIX_NVARIANTS] = {{16, 32, 256}, {8, 64, 256}, {8, 32, 256},
+                                                 {16, 64, 512}};
 
 hipError_t launch_stage_mixed(int variant, const CaseMix *cases, const TileDesc *tiles, int ntiles,
                               int grid, int stage, double alpha, bool terms, hipStream_t s)
@@ -702,14 +747,20 @@ hipError_t launch_stage_mixed(int variant, const CaseMix *cases, const TileDesc 
     case 0: return stage_launch<16, 32, 256>(cases, tiles, ntiles, grid, stage, alpha, terms, s);
     case 1: return stage_launch<8, 64, 256>(cases, tiles, ntiles, grid, stage, alpha, terms, s);
     case 2: return stage_launch<8, 32, 256>(cases, tiles, ntiles, grid, stage, alpha, terms, s);
+    case 3: return stage_launch<16, 64, 512>(cases, tiles, ntiles, grid, stage, alpha, terms, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_post_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, int mode,
-                             hipStream_t s)
+hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s)
 {
-    hipLaunchKernelGGL((mix_post_kernel<256>), dim3(nposts), dim3(256), 0, s, cases, posts, mode);
+    hipLaunchKernelGGL((mix_end_kernel<256>), dim3(nslots), dim3(256), 0, s, cases, slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, hipStream_t s)
+{
+    hipLaunchKernelGGL((mix_dt_kernel<256>), dim3(nposts), dim3(256), 0, s, cases, posts);
     return hipGetLastError();
 }
 

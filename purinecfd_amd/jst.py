@@ -36,7 +36,7 @@ EXPORTS = ["jst_last_error", "jst_config_check", "jst_mesh_parse", "jst_create",
            "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces",
            "jst_batch_get_field", "jst_batch_residual_terms", "jst_batch_history",
            "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_steps", "jst_batch_counts",
-           "jst_serve"]
+           "jst_copy_probe", "jst_serve"]
 JST_ERR_NOSPACE = 11
 CASE_RUNNING, CASE_CONVERGED, CASE_STOPPED, CASE_NONPHYSICAL = 0, 1, 2, 3
 
@@ -151,6 +151,7 @@ def lib():
         L.jst_batch_tune.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
         L.jst_batch_time_steps.argtypes = [vp, ip, dp, dp]
         L.jst_batch_counts.argtypes = [vp, C.POINTER(ip), C.POINTER(C.c_long)]
+        L.jst_copy_probe.argtypes = [sz, dp]
         L.jst_serve.argtypes = [C.POINTER(_Request), ip, C.POINTER(_ServeOpts),
                                 C.POINTER(_RequestResult), C.POINTER(_ServeSummary)]
         for name in EXPORTS[1:]:
@@ -470,3 +471,10 @@ def serve(requests, hbm_bytes: int = 16 << 30, max_cases: int = 64, steps_per_ro
     return out, {"wall_s": summ.wall_s, "cell_stages": summ.cell_stages, "p50_s": summ.p50_s,
                  "p99_s": summ.p99_s, "completed": summ.completed,
                  "nonphysical": summ.nonphysical, "max_concurrent": summ.max_concurrent}
+
+
+def copy_probe(nbytes: int) -> float:
+    """Device-to-device copy of nbytes (profiler byte-counter calibration); returns ms."""
+    ms = C.c_double()
+    _check(lib().jst_copy_probe(int(nbytes), C.byref(ms)))
+    return ms.value

@@ -113,7 +113,7 @@ def test_batch_alone_equals_in_the_mix_bitwise(jst):
 def test_tile_variant_and_grid_do_not_change_results(jst):
     cfg, text = _cases()[1]
     ref = None
-    for variant, grid in ((0, 0), (1, 0), (2, 0), (0, 512), (2, 1024)):
+    for variant, grid in ((0, 0), (1, 0), (2, 0), (3, 0), (0, 512), (2, 1024)):
         b = jst.Batch(1 << 30, 2)
         b.set_tiling(variant, grid)
         cid = b.add(cfg, text, 40)
@@ -138,8 +138,8 @@ def test_graph_replays_match_eager_launches(jst):
     h1 = b1.history(c1, 1, 30)
     b2 = jst.Batch(1 << 30, 2)
     c2 = b2.add(cfg, text, 30)
-    for _ in range(3):
-        b2.step(10)
+    for n in (16, 10, 4):  # a 16-step graph, then single-step graphs
+        b2.step(n)
     h2 = b2.history(c2, 1, 30)
     assert np.array_equal(h1, h2)
 
