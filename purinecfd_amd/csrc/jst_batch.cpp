@@ -79,7 +79,7 @@ struct Arena {
 
 // Byte layout of one case inside its arena block.
 struct CaseLayout {
-    size_t u64[2], s32[2], g64[14], g32[7], ratio[2], localdt[2], rpart, dpart, res_hist, posts, scratch, dbg;
+    size_t u64[2], sb, s32[2], g64[14], g32[7], ratio[2], localdt[2], rpart, dpart, res_hist, posts, scratch, dbg;
     size_t total;
 };
 
@@ -94,6 +94,7 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
         return a;
     };
     for (int b = 0; b < 2; ++b) L.u64[b] = carve(5 * plane * 8);
+    L.sb = carve(5 * plane * 4);
     for (int b = 0; b < 2; ++b) L.s32[b] = carve(5 * plane * 4);
     // FP64 geometry: vol cx cy wd inv_vol inv_wd2 (plane) | tnx tny tmx tmy (fplane) | nnx nny nmx nmy
     for (int q = 0; q < 6; ++q) L.g64[q] = carve(plane * 8);
@@ -104,8 +105,8 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
-    L.rpart = carve((size_t)ni * (pitch / 32) * 8);
-    L.dpart = carve((size_t)ni * (pitch / 32) * 8);
+    L.rpart = carve((size_t)ni * ((nj + 7) / 8) * 8);
+    L.dpart = carve((size_t)ni * ((nj + 7) / 8) * 8);
     L.res_hist = carve((size_t)res_cap * 8);
     L.posts = carve((size_t)npost * sizeof(PostDesc));
     L.scratch = carve((2 * (size_t)nj + 2) * 8);
@@ -142,6 +143,8 @@ MixConst make_mix_const(const Config &c, const Derived &d)
     m.gamma = (float)c.gamma;
     m.gm1 = (float)(c.gamma - 1.0);
     m.r_gas = (float)c.r_gas;
+    m.cfl = (float)c.cfl;
+    m.inv_cfl = (float)(1.0 / c.cfl);
     m.cfl_half = (float)(0.5 * c.cfl);
     m.k2 = (float)c.k2;
     m.k4 = (float)c.k4;
@@ -239,27 +242,34 @@ jst_status drop_graph(jst_batch *b)
     return JST_OK;
 }
 
-// Rebuild the tile list (XCD-aware: consecutive tiles of a case are dealt to the same XCD,
-// since workgroup w runs on XCD w % 8) and the step-end slot list.
+// Work items of one case for the row march: 56-column groups x row segments of `rows` rows.
+void case_items(const HostCase &hc, int rows, std::vector<TileDesc> &out)
+{
+    for (int j0 = 0; j0 < hc.nj; j0 += MIX_COLS)
+        for (int i0 = 0; i0 < hc.ni; i0 += rows)
+            out.push_back(TileDesc{hc.slot, i0, j0, std::min(i0 + rows, hc.ni)});
+}
+
+// Rebuild the work-item list and the step-end slot list.  XCD-aware: workgroup w (four
+// consecutive items) runs on XCD w % 8, so runs of consecutive items of a case - neighbouring
+// row segments, which share halo rows - are dealt to one XCD.
 jst_status rebuild_lists(jst_batch *b)
 {
-    const jst::MixVariant v = jst::MIX_VARIANTS[b->variant];
+    const int rows = jst::MIX_SEGMENT_ROWS[b->variant];
     std::vector<TileDesc> nat;
     std::vector<int> slots;
     for (auto &hc : b->cases) {
         if (!hc) continue;
-        const int nti = (hc->ni + v.ti - 1) / v.ti, ntj = (hc->nj + v.tj - 1) / v.tj;
-        for (int a = 0; a < nti; ++a)
-            for (int c = 0; c < ntj; ++c) nat.push_back(TileDesc{hc->slot, a * v.ti, c * v.tj, a * ntj + c});
+        case_items(*hc, rows, nat);
         slots.push_back(hc->slot);
     }
-    const int n = (int)nat.size(), per = (n + 7) / 8;
+    const int nu = ((int)nat.size() + 3) / 4, per = (nu + 7) / 8; // units of 4 items
     std::vector<TileDesc> tiles;
-    tiles.reserve(n);
+    tiles.reserve(nat.size());
     for (int q = 0; q < per; ++q)
         for (int x = 0; x < 8; ++x) {
-            int t = x * per + q;
-            if (t < n) tiles.push_back(nat[t]);
+            const int u = x * per + q;
+            for (int e = 0; e < 4 && u < nu && 4 * u + e < (int)nat.size(); ++e) tiles.push_back(nat[4 * u + e]);
         }
     if (tiles.size() > b->tiles_cap) {
         if (b->d_tiles) HIPCHK(hipFree(b->d_tiles));
@@ -286,8 +296,8 @@ jst_status rebuild_lists(jst_batch *b)
 jst_status launch_one_step(jst_batch *b)
 {
     for (int k = 0; k < 5; ++k)
-        HIPCHK(jst::launch_stage_mixed(b->variant, b->d_cases, b->d_tiles, b->ntiles, b->grid, k,
-                                       RK_ALPHA[k], false, b->stream));
+        HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k], false,
+                                       b->stream));
     HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
     return JST_OK;
 }
@@ -543,6 +553,10 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     st = parse_mesh(mesh_text, mesh);
     if (st != JST_OK) return st;
     if (hc->cfg.halo != JST_H) return fail(JST_ERR_CONFIG, "unsupported halo: kernels are built for IM=3");
+    if (mesh.n_rings - 1 < JST_H || mesh.n_theta < 4)
+        return fail(JST_ERR_ARG, "the mixed path needs at least 3 cell rings and 4 cells per ring");
+    if ((size_t)mesh.n_rings * ((mesh.n_theta + 31) / 32 * 32) * 6 >= ((size_t)1 << 31))
+        return fail(JST_ERR_ARG, "mesh too large for 32-bit plane offsets");
     int slot = -1;
     for (int q = 0; q < b->max_cases; ++q)
         if (!b->cases[q]) {
@@ -573,7 +587,8 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     cm.nj = hc->nj;
     cm.pitch = hc->pitch;
     cm.npost = hc->npost;
-    cm.nchunk = hc->pitch / 32;
+    cm.nchunk = (hc->nj + 7) / 8; // every chunk holds at least one column: all are written
+    cm.sb = F(L.sb);
     for (int q = 0; q < 2; ++q) {
         cm.u64[q] = D(L.u64[q]);
         cm.s32[q] = F(L.s32[q]);
@@ -735,17 +750,13 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out)
 {
     HostCase *hc = get_case(b, case_id);
     if (!hc) return fail(JST_ERR_ARG, "no such case");
-    const jst::MixVariant v = jst::MIX_VARIANTS[b->variant];
     std::vector<TileDesc> t;
-    const int nti = (hc->ni + v.ti - 1) / v.ti, ntj = (hc->nj + v.tj - 1) / v.tj;
-    for (int a = 0; a < nti; ++a)
-        for (int c = 0; c < ntj; ++c) t.push_back(TileDesc{case_id, a * v.ti, c * v.tj, a * ntj + c});
+    case_items(*hc, jst::MIX_SEGMENT_ROWS[b->variant], t);
     TileDesc *dt = nullptr;
     HIPCHK(hipMalloc(&dt, t.size() * sizeof(TileDesc)));
     HIPCHK(hipMemcpyAsync(dt, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice, b->stream));
     HIPCHK(jst::launch_dt_mixed(b->d_cases, hc->d_posts, hc->npost, b->stream));
-    HIPCHK(jst::launch_stage_mixed(b->variant, b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true,
-                                   b->stream));
+    HIPCHK(jst::launch_stage_mixed(b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true, b->stream));
     jst_status st = fetch_ctl(b);
     HIPCHK(hipFree(dt));
     if (st != JST_OK) return st;
@@ -865,8 +876,8 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
         for (int s = 0; s < nsteps; ++s) {
             for (int k = 0; k < 5; ++k) {
                 HIPCHK(hipEventRecord(evs[10 * s + 2 * k], b->stream));
-                HIPCHK(jst::launch_stage_mixed(b->variant, b->d_cases, b->d_tiles, b->ntiles, b->grid, k,
-                                               RK_ALPHA[k], false, b->stream));
+                HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
+                                               false, b->stream));
                 HIPCHK(hipEventRecord(evs[10 * s + 2 * k + 1], b->stream));
             }
             HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));

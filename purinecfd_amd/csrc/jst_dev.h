@@ -81,7 +81,7 @@ enum {
 // ---------------------------------------------------------------------------------------
 struct MixConst {
     // FP32 scheme constants
-    float gamma, gm1, r_gas, cfl_half, k2, k4;
+    float gamma, gm1, r_gas, cfl, inv_cfl, cfl_half, k2, k4;
     float rho_inf, u_inf, v_inf, p_inf, t_inf, m_inf, n_inf, e_inf, h_inf, rnu_inf, nut_inf;
     float suth_num, inv_t_ref, t_suth, inv_pr, inv_prt, cp, sigma, cv1_cubed;
     float cb1, cb2, ct3, ct4, cw1, cw2, cw3_6, one_plus_cw3_6, fv3, inv_kappa2, rmax;
@@ -94,16 +94,17 @@ struct MixConst {
 struct CaseMix {
     int ni, nj, pitch;
     int npost;                    // dt-refresh blocks of this case (fixed 1024-cell chunks)
-    int nchunk;                   // 32-column chunks per row (pitch / 32)
+    int nchunk;                   // 8-column chunks per row, ceil(nj / 8)
     double *u64[2];               // U^n / U^{n+1}, 5 planes each
-    float *s32[2];                // FP32 shadow dU, 5 planes each (stage ping-pong)
+    float *sb;                    // FP32 base dU^n = U^n - U_inf, 5 planes
+    float *s32[2];                // FP32 stage states dU^(k), 5 planes each (ping-pong)
     const float *tnx, *tny;       // FP32 tau normals (ni+1) x pitch
     const float *nnx, *nny;       // FP32 n normals
     const float *vol, *inv_vol, *inv_wd2;
     const double *tnx64, *tny64, *tmx64, *tmy64, *cx64, *cy64; // far-field support (FP64)
     float *ratio[2], *localdt[2]; // frozen per step; [rot] is read by the step, [rot^1] written
-    double *rpart;                // residual partial sums per (row, 32-column chunk)
-    double *dpart;                // next-dt minima per (row, 32-column chunk)
+    double *rpart;                // residual partial sums per (row, 8-column chunk)
+    double *dpart;                // next-dt minima per (row, 8-column chunk)
     double *res_hist;
     int res_cap;
     CaseCtl *ctl;

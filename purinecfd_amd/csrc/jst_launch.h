@@ -27,15 +27,13 @@ hipError_t launch_post_fp64(const CaseDev *cases, const TileDesc *tiles, int nti
 hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, double *out,
                               hipStream_t s);
 
-// FP32-mixed stage kernel variants (tile rows x tile columns x threads); the autotuner picks
-// one per batch.  Results do not depend on the variant.
-struct MixVariant {
-    int ti, tj, nt;
-};
+// FP32-mixed row-march variants: rows per work item (one wave per 56 columns x that many
+// rows); the autotuner picks one per batch.  Results do not depend on the variant.
 #define MIX_NVARIANTS 4
-extern const MixVariant MIX_VARIANTS[MIX_NVARIANTS];
-hipError_t launch_stage_mixed(int variant, const CaseMix *cases, const TileDesc *tiles, int ntiles,
-                              int grid, int stage, double alpha, bool terms, hipStream_t s);
+#define MIX_COLS 56
+extern const int MIX_SEGMENT_ROWS[MIX_NVARIANTS];
+hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
+                              int stage, double alpha, bool terms, hipStream_t s);
 hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s);
 hipError_t launch_dt_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, hipStream_t s);
 hipError_t launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);

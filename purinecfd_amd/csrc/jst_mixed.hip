@@ -11,6 +11,8 @@
 // gradient.rs, viscous.rs, dissipation.rs, source.rs, solver.rs:123-200 (update, residual).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include <algorithm>
 
 #include "jst_dev.h"
@@ -215,375 +217,419 @@ __device__ __forceinline__ float sensor32(float pinf4, float dm, float d0, float
     return fabsf((dm - 2.0f * d0 + dp) * frcp(pinf4 + dm + 2.0f * d0 + dp));
 }
 
-template <int TI, int TJ>
-struct MixSmem {
-    static constexpr int H3 = TI + 6, W3 = TJ + 6, H1 = TI + 2, W1 = TJ + 2;
-    union {
-        struct {
-            float u[5][H3][W3];
-            float p[H3][W3]; // dp
-            float du[H3][W3];
-            float dv[H3][W3];
-            float dT[H3][W3];
-            float nu[H3][W3];
-        } a;
-        struct {
-            float gt[5][TI + 1][TJ]; // tau faces i0 .. i0+TI
-            float gn[5][TI][TJ + 1]; // n faces j0 .. j0+TJ
-        } f;
-    };
-    float si[TI + 4][TJ];
-    float sj[TI][TJ + 4];
-    float ratio[H1][W1];
-    float d[8][H1][W1];
-    float src[TI][TJ];
-    int col[W3]; // wrapped column index of R3 column c (j0 - 3 + c mod NJ)
-};
+// Lane l of a wave owns column J0 + l - 4 for 4 <= l < 60 (56 columns); lanes 0..3 and 60..63
+// carry the periodic halo.  Values of the j-neighbours move between lanes with DPP wave shifts,
+// which read every lane, so they are only ever called from wave-uniform control flow.
+__device__ __forceinline__ float from_left(float x) // lane l receives lane l-1
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float x) // lane l receives lane l+1
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+}
+
+// One face: perturbation Euler flux of the face average, JST dissipation with the 4-cell
+// stencil um2 | um1 || u0 | up1 and the sensors of those cells, viscous flux from the
+// diffusion tensors of the two cells (jst-rs convection.rs, dissipation.rs, viscous.rs).
+// G = dFc - Fv - Fd.
+__device__ __forceinline__ void face_flux(const MixConst &k, const float um2[5], const float um1[5],
+                                          const float u0[5], const float up1[5], float s0, float s1,
+                                          float s2, float s3, float lam, const float dA[8],
+                                          const float dB[8], float nx, float ny, float G[5])
+{
+    lam *= k.cfl_half;
+    const float mx = fmaxf(fmaxf(fmaxf(s0, s1), s2), s3);
+    const float e2 = k.k2 * mx;
+    const float e4 = fmaxf(k.k4 - e2, 0.0f);
+    float fc[5];
+    euler_flux_pert(k, um1, u0, nx, ny, fc);
+    const float hx = 0.5f * nx, hy = 0.5f * ny;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const float d1 = u0[m] - um1[m];
+        const float d3 = up1[m] - 3.0f * u0[m] + 3.0f * um1[m] - um2[m];
+        const float fd = (d1 * e2 - d3 * e4) * lam;
+        const float fv = m == 0 ? 0.0f : (dA[m - 1] + dB[m - 1]) * hx + (dA[m + 3] + dB[m + 3]) * hy;
+        G[m] = fc[m] - fv - fd;
+    }
+}
+
+// Sum (or minimum) of the 8 lanes 4+8q .. 11+8q, left in lane 4+8q, in a fixed order.
+__device__ __forceinline__ double seg8_sum(double x)
+{
+    x += __shfl_down(x, 1, 64);
+    x += __shfl_down(x, 2, 64);
+    x += __shfl_down(x, 4, 64);
+    return x;
+}
+__device__ __forceinline__ double seg8_min(double x)
+{
+    x = fmin(x, __shfl_down(x, 1, 64));
+    x = fmin(x, __shfl_down(x, 2, 64));
+    x = fmin(x, __shfl_down(x, 4, 64));
+    return x;
+}
+
+enum { MIX_INNER = 0, MIX_LAST = 1, MIX_TERMS = 2 };
 
 } // namespace
 
 // ---------------------------------------------------------------------------------------
-// Fused FP32 stage, grid-strided over a tile list spanning any number of cases.  Stage 5
-// also computes the step-end quantities of its own cells: the next local dt and V/dt (into
-// the other half of their double buffer) and, per row and 32-column chunk, the residual
-// partial sum and the dt minimum.  Partials are keyed by cell position, not by tile, so the
-// step end does not depend on the tile shape or on the other cases of the batch.
+// FP32-mixed RK stage as a row march.  One wave per work item (case, 56-column group, row
+// segment [is, ie)): lanes hold columns, the wave walks rows is-3 .. ie+2 keeping short
+// windows of rows in registers (state, pressure, primitives, sensors, diffusion tensors,
+// frozen V/dt), with the next row's loads issued one step ahead.  After loading row r it
+//   - forms the primitives of row r (wall mirror / far-field Riemann ghosts synthesised),
+//   - the i-sensor of row r-1 and the j-sensor of row r,
+//   - the Green-Gauss gradient, diffusion tensor and SA source of row r-1,
+//   - the tau face r-2 (between rows r-3 and r-2),
+//   - the n faces and the residual of cell row r-3, and updates it.
+// Every face is evaluated once; the right n face of a lane is its neighbour's left face.
+// MIX_INNER (stages 1-4): U^(k) = U^n - a_k dt/V R in FP32 from the FP32 base dU^n.
+// MIX_LAST (stage 5): FP64 U^{n+1} = U^n - dt/V R, the new FP32 base, the next frozen dt
+// and V/dt, and the step-end partials per (row, 8-column chunk).
+// MIX_TERMS: export R = Fc - Fv - Fd and S of the current state (no update).
 // ---------------------------------------------------------------------------------------
-template <int TI, int TJ, int NT, bool TERMS>
-__global__ __launch_bounds__(NT) void mix_stage_kernel(const CaseMix *__restrict__ cases,
-                                                       const TileDesc *__restrict__ tiles,
-                                                       int ntiles, int stage, double alpha)
+template <int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mix_march_kernel(const CaseMix *__restrict__ cases,
+                                                        const TileDesc *__restrict__ items, int nitems,
+                                                        int stage, double alpha)
 {
-    using S = MixSmem<TI, TJ>;
-    constexpr int H3 = S::H3, W3 = S::W3, H1 = S::H1, W1 = S::W1;
-    constexpr int NFT = (TI + 1) * TJ, NFN = TI * (TJ + 1);
-    constexpr int FPT = (NFT + NFN + NT - 1) / NT;
-    static_assert(TJ % 32 == 0 && (TI * TJ) % NT == 0 && NT % 64 == 0, "tile shape");
-    __shared__ S sm;
-    const int tid = threadIdx.x;
-    const bool last_stage = !TERMS && stage == 4;
-
-    for (int tix = blockIdx.x; tix < ntiles; tix += gridDim.x) {
-        const TileDesc tl = tiles[tix];
-        const CaseMix &C = cases[tl.c];
+    const int lane = threadIdx.x & 63;
+    const int nwaves = gridDim.x * 4;
+    for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nitems; it += nwaves) {
+        const TileDesc w = items[it];
+        const CaseMix &C = cases[w.c];
         const MixConst &k = C.k;
         CaseCtl *ctl = C.ctl;
         const int ni = C.ni, nj = C.nj, pitch = C.pitch;
         const int plane = ni * pitch;
-        const int i0 = tl.i0, j0 = tl.j0;
+        const int is = w.i0, J0 = w.j0, ie = w.t;
         const int status = *(volatile int *)&ctl->status;
         const long long step = *(volatile long long *)&ctl->step;
         const unsigned long long ek = *(volatile unsigned long long *)&ctl->err_key;
         const int rot = *(volatile int *)&ctl->rot;
         const long long seq = step * 5 + stage;
-        if (!TERMS) {
-            if (status != JST_RUN_ACTIVE) continue; // uniform per block
+        if (MODE != MIX_TERMS) {
+            if (status != JST_RUN_ACTIVE) continue;                          // wave-uniform
             if (ek != JST_ERR_NONE && (long long)(ek >> 32) < seq) continue; // failed earlier
         }
-        const float *src = TERMS ? C.s32[rot] : C.s32[rot ^ (stage & 1)];
-        float *sdst = C.s32[rot ^ ((stage & 1) ^ 1)];
-        const double *former = C.u64[rot];
-        double *u_next = C.u64[rot ^ 1];
+        const float *src = (MODE == MIX_TERMS || stage == 0) ? C.sb : C.s32[(stage - 1) & 1];
+        float *sdst = C.s32[stage & 1];
         const float *ratio_now = C.ratio[rot];
-        const double dt = ctl->dt;
-        const double inv_cfl = 1.0 / k.k.cfl;
+        const double a_dt = alpha * ctl->dt;
 
-        for (int c = tid; c < W3; c += NT) {
-            int jw = j0 - JST_H + c;
-            while (jw < 0) jw += nj;
-            while (jw >= nj) jw -= nj;
-            sm.col[c] = jw;
-        }
-        __syncthreads();
+        const int jg = J0 - 4 + lane; // unwrapped column of this lane
+        int jw = jg;
+        while (jw < 0) jw += nj;
+        while (jw >= nj) jw -= nj;
+        const bool own = lane >= 4 && lane < 60 && jg < nj;
 
-        // ---- phase 1: R3 load + ghost synthesis (perturbation variables) ----
-        for (int idx = tid; idx < H3 * W3; idx += NT) {
-            const int r = idx / W3, c = idx - r * W3;
-            const int i = i0 - JST_H + r; // >= -3
-            float d[5], dp, du, dv, dT, nut;
-            if (i < ni) {
-                // interior row i, or row -1-i for the wall mirror (boundary.rs:30-51)
-                const int q = cidx(i >= 0 ? i : -1 - i, sm.col[c], pitch);
+        // register windows (index 0 = oldest row)
+        float U[5][5];     // state dU, rows r-4 .. r
+        float P[3];        // dp, rows r-2 .. r
+        float PR[3][4];    // du dv dT nu~, rows r-2 .. r
+        float SI[4];       // i-direction sensor, rows r-4 .. r-1
+        float SJ[4];       // j-direction sensor, rows r-3 .. r
+        float RT[4];       // frozen V/dt (row clamped), rows r-3 .. r
+        float D[3][8];     // diffusion tensor, rows r-3 .. r-1
+        float S[3];        // SA source, rows r-3 .. r-1
+        float TN[4][2];    // tau-face normals, faces r-3 .. r
+        float NN[3][2];    // n-face normals (left face of the lane's cell), rows r-3 .. r-1
+        float IV[3], VOL[3]; // 1/V and V, rows r-3 .. r-1
+        float GTp[5];      // tau face r-3 (previous step)
+        // bundle of row r, loaded during step r-1
+        float bu[5] = {}, brt = 0.f, btn[2] = {}, bnn[2] = {}, biv = 0.f, bvol = 0.f, biw = 0.f;
+        float bsb[5] = {}, bldt = 0.f;
+        double bf[5] = {};
+
+        // Loads of row r's bundle.  In a steady step every condition below holds except that
+        // row r may be the first far-field ghost row: its (unused) state load is clamped.
+        auto load_bundle = [&](int r, auto steady) {
+            constexpr bool ST = decltype(steady)::value;
+            if (ST || r < ni) {
+                const int rr = ST ? (r < ni ? r : ni - 1) : (r >= 0 ? r : -1 - r); // -1-r: wall mirror
+                const int q = cidx(rr, jw, pitch);
 #pragma unroll
-                for (int m = 0; m < 5; ++m) d[m] = src[m * plane + q];
-                prim_from_du(k, d, dp, du, dv, dT, nut);
-                if (i < 0) { // scalars copied, u, v, nu~ negated
-                    d[1] = -d[1] - 2.0f * k.m_inf;
-                    d[2] = -d[2] - 2.0f * k.n_inf;
-                    d[4] = -d[4] - 2.0f * k.rnu_inf;
+                for (int m = 0; m < 5; ++m) bu[m] = src[m * plane + q];
+            }
+            const int rc = r < 0 ? 0 : (r > ni - 1 ? ni - 1 : r);
+            brt = ratio_now[cidx(rc, jw, pitch)];
+            if (ST || (r >= 0 && r <= ni)) {
+                btn[0] = C.tnx[cidx(r, jw, pitch)];
+                btn[1] = C.tny[cidx(r, jw, pitch)];
+            }
+            const int x = r - 1;
+            if (ST || (x >= 0 && x < ni)) {
+                const int q = cidx(x, jw, pitch);
+                bnn[0] = C.nnx[q];
+                bnn[1] = C.nny[q];
+                biv = C.inv_vol[q];
+                bvol = C.vol[q];
+                biw = C.inv_wd2[q];
+            }
+            const int c = r - 3;
+            if (MODE != MIX_TERMS && (ST || (c >= is && c < ie))) {
+                const int q = cidx(c, jw, pitch);
+                if (MODE == MIX_INNER) {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) bsb[m] = C.sb[m * plane + q];
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) bf[m] = C.u64[rot][m * plane + q];
+                }
+                if (k.lts) bldt = C.localdt[rot][q];
+            }
+        };
+
+        // One row step.  steady = true: interior row with every stage of the pipeline active
+        // (no ghost rows, no boundary special cases), compiled without any branch.
+        auto row_step = [&](int r, auto steady) {
+            constexpr bool ST = decltype(steady)::value;
+            // ---- take the bundle of row r, prefetch row r+1 ----
+            float un[5], rt_new = brt, tn_new[2] = {btn[0], btn[1]}, nn_new[2] = {bnn[0], bnn[1]};
+            const float iv_new = biv, vol_new = bvol, iw_new = biw;
+            float base[5];
+            double former[5];
+            float ldt_c = bldt;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                un[m] = bu[m];
+                base[m] = bsb[m];
+                former[m] = bf[m];
+            }
+            if (ST || r < ie + 2) load_bundle(r + 1, steady);
+
+            // ---- row r: state and primitives, ghosts synthesised ----
+            float dp, du, dv, dT, nut;
+            if (ST || r < ni) {
+                prim_from_du(k, un, dp, du, dv, dT, nut);
+                if (!ST && r < 0) { // wall mirror (boundary.rs:30-51): scalars copied, u, v, nu~ negated
+                    un[1] = -un[1] - 2.0f * k.m_inf;
+                    un[2] = -un[2] - 2.0f * k.n_inf;
+                    un[4] = -un[4] - 2.0f * k.rnu_inf;
                     du = -du - 2.0f * k.u_inf;
                     dv = -dv - 2.0f * k.v_inf;
                     nut = -nut;
                 }
-            } else if (i < ni + JST_H) {
-                double s[7];
-                far_state64(C, src, sm.col[c], s);
-                const double *U = C.k.u64;
-                d[0] = (float)(s[0] - U[0]);
-                d[1] = (float)(s[0] * s[3] - U[1]);
-                d[2] = (float)(s[0] * s[4] - U[2]);
-                d[3] = (float)(s[0] * s[5] - U[3]);
-                d[4] = (float)(s[0] * s[6] - U[4]);
-                dp = (float)(s[1] - C.k.k.p_inf);
-                du = (float)(s[3] - C.k.k.u_inf);
-                dv = (float)(s[4] - C.k.k.v_inf);
-                dT = (float)(s[2] - C.k.k.t_inf);
-                nut = (float)s[6];
-            } else {
-#pragma unroll
-                for (int m = 0; m < 5; ++m) d[m] = 0.0f;
-                dp = du = dv = dT = 0.0f;
-                nut = k.nut_inf;
+            } else { // far-field ghost rows ni .. ni+2 (boundary.rs:112-185)
+                double fs[7];
+                far_state64(C, src, jw, fs);
+                const double *Ui = C.k.u64;
+                un[0] = (float)(fs[0] - Ui[0]);
+                un[1] = (float)(fs[0] * fs[3] - Ui[1]);
+                un[2] = (float)(fs[0] * fs[4] - Ui[2]);
+                un[3] = (float)(fs[0] * fs[5] - Ui[3]);
+                un[4] = (float)(fs[0] * fs[6] - Ui[4]);
+                dp = (float)(fs[1] - C.k.k.p_inf);
+                du = (float)(fs[3] - C.k.k.u_inf);
+                dv = (float)(fs[4] - C.k.k.v_inf);
+                dT = (float)(fs[2] - C.k.k.t_inf);
+                nut = (float)fs[6];
             }
 #pragma unroll
-            for (int m = 0; m < 5; ++m) sm.a.u[m][r][c] = d[m];
-            sm.a.p[r][c] = dp;
-            sm.a.du[r][c] = du;
-            sm.a.dv[r][c] = dv;
-            sm.a.dT[r][c] = dT;
-            sm.a.nu[r][c] = nut;
-        }
-        for (int idx = tid; idx < H1 * W1; idx += NT) {
-            const int r = idx / W1, c = idx - r * W1;
-            int i = i0 - 1 + r;
-            i = i < 0 ? 0 : (i > ni - 1 ? ni - 1 : i);
-            sm.ratio[r][c] = ratio_now[cidx(i, sm.col[c + 2], pitch)];
-        }
-        __syncthreads();
+            for (int m = 0; m < 5; ++m) U[4][m] = un[m];
+            P[2] = dp;
+            PR[2][0] = du;
+            PR[2][1] = dv;
+            PR[2][2] = dT;
+            PR[2][3] = nut;
+            RT[3] = rt_new;
+            TN[3][0] = tn_new[0];
+            TN[3][1] = tn_new[1];
 
-        // ---- phase 2: sensors, gradient + viscous tensor on R1, SA source ----
-        const float pinf4 = 4.0f * k.p_inf;
-        for (int idx = tid; idx < (TI + 4) * TJ; idx += NT) {
-            const int r = idx / TJ, c = idx - r * TJ;
-            sm.si[r][c] = sensor32(pinf4, sm.a.p[r][c + 3], sm.a.p[r + 1][c + 3], sm.a.p[r + 2][c + 3]);
-        }
-        for (int idx = tid; idx < TI * (TJ + 4); idx += NT) {
-            const int r = idx / (TJ + 4), c = idx - r * (TJ + 4);
-            sm.sj[r][c] = sensor32(pinf4, sm.a.p[r + 3][c], sm.a.p[r + 3][c + 1], sm.a.p[r + 3][c + 2]);
-        }
-        for (int idx = tid; idx < H1 * W1; idx += NT) {
-            const int r = idx / W1, c = idx - r * W1;
-            const int i = i0 - 1 + r, jj = j0 - 1 + c;
-            const int R = r + 2, Q = c + 2;
-            if (i > ni) continue;
-            float g[8];
-            if (i == ni) {
+            // ---- sensors: i-direction of row r-1, j-direction of row r ----
+            const float pinf4 = 4.0f * k.p_inf;
+            SI[3] = sensor32(pinf4, P[0], P[1], P[2]);
+            SJ[3] = sensor32(pinf4, from_left(P[2]), P[2], from_right(P[2]));
+
+            // ---- row x = r-1: gradient, diffusion tensor, SA source ----
+            const int x = r - 1;
+            NN[2][0] = nn_new[0];
+            NN[2][1] = nn_new[1];
+            IV[2] = iv_new;
+            VOL[2] = vol_new;
+            if (ST || (x >= (is - 1 > 0 ? is - 1 : 0) && x <= ie)) {
+                float g[8];
+                if (!ST && x == ni) {
 #pragma unroll
-                for (int m = 0; m < 8; ++m) g[m] = 0.0f;
-            } else {
-                const int ie = i < 0 ? 0 : i;
-                const int Re = i < 0 ? R + 1 : R;
-                const int q0 = cidx(ie, sm.col[Q], pitch), qp = cidx(ie, sm.col[Q + 1], pitch);
-                const float upx = C.tnx[q0 + pitch], upy = C.tny[q0 + pitch];
-                const float dnx = C.tnx[q0], dny = C.tny[q0];
-                const float lfx = C.nnx[q0], lfy = C.nny[q0];
-                const float rtx = C.nnx[qp], rty = C.nny[qp];
-                const float inv_v = C.inv_vol[q0];
+                    for (int m = 0; m < 8; ++m) g[m] = 0.0f;
+                } else {
+                    const float upx = TN[3][0], upy = TN[3][1], dnx = TN[2][0], dny = TN[2][1];
+                    const float lfx = NN[2][0], lfy = NN[2][1];
+                    const float rtx = from_right(lfx), rty = from_right(lfy);
 #pragma unroll
-                for (int f = 0; f < 4; ++f) {
-                    const float(*fld)[W3] = f == 0 ? sm.a.du : (f == 1 ? sm.a.dv : (f == 2 ? sm.a.dT : sm.a.nu));
-                    const float cc = fld[Re][Q];
-                    const float f_up = 0.5f * (cc + fld[Re + 1][Q]);
-                    const float f_dn = 0.5f * (cc + fld[Re - 1][Q]);
-                    const float f_lf = 0.5f * (cc + fld[Re][Q - 1]);
-                    const float f_rt = 0.5f * (cc + fld[Re][Q + 1]);
-                    g[2 * f] = (f_up * upx - f_dn * dnx + f_rt * rtx - f_lf * lfx) * inv_v;
-                    g[2 * f + 1] = (f_up * upy - f_dn * dny + f_rt * rty - f_lf * lfy) * inv_v;
+                    for (int f = 0; f < 4; ++f) {
+                        const float cc = PR[1][f];
+                        const float f_up = 0.5f * (cc + PR[2][f]);
+                        const float f_dn = 0.5f * (cc + PR[0][f]);
+                        const float f_lf = 0.5f * (cc + from_left(cc));
+                        const float f_rt = 0.5f * (cc + from_right(cc));
+                        g[2 * f] = (f_up * upx - f_dn * dnx + f_rt * rtx - f_lf * lfx) * IV[2];
+                        g[2 * f + 1] = (f_up * upy - f_dn * dny + f_rt * rty - f_lf * lfy) * IV[2];
+                    }
                 }
-                if (i < 0) {
+                float chi, fv1;
+                diffusion_tensor32(k, PR[1][2], U[3][4], PR[1][0], PR[1][1], g, D[2], chi, fv1);
+                if (ST || (x >= is && x < ie && x < ni))
+                    S[2] = sa_source32(k, U[3][0], U[3][4], chi, fv1, g, iw_new, VOL[2]);
+                if (!ST && x == 0) {
+                    // ghost row -1 takes the gradient of row 0 without its temperature part
                     g[4] = 0.0f;
                     g[5] = 0.0f;
+                    float chi_g, fv1_g;
+                    diffusion_tensor32(k, PR[0][2], U[2][4], PR[0][0], PR[0][1], g, D[1], chi_g, fv1_g);
                 }
             }
-            float d[8], chi, fv1;
-            diffusion_tensor32(k, sm.a.dT[R][Q], sm.a.u[4][R][Q], sm.a.du[R][Q], sm.a.dv[R][Q], g, d,
-                               chi, fv1);
-#pragma unroll
-            for (int m = 0; m < 8; ++m) sm.d[m][r][c] = d[m];
-            if (r >= 1 && r <= TI && c >= 1 && c <= TJ && i < ni && jj < nj) {
-                const int q = cidx(i, jj, pitch);
-                sm.src[r - 1][c - 1] = sa_source32(k, sm.a.u[0][R][Q], sm.a.u[4][R][Q], chi, fv1, g,
-                                                   C.inv_wd2[q], C.vol[q]);
-            }
-        }
-        __syncthreads();
 
-        // ---- phase 3: every face of the tile once: G = dFc - Fv - Fd ----
-        float G[FPT][5];
-        int gsel[FPT];
-#pragma unroll
-        for (int q = 0; q < FPT; ++q) {
-            const int idx = tid + q * NT;
-            gsel[q] = -1;
-            if (idx >= NFT + NFN) continue;
-            float lam, e2, e4, nx, ny, dA[8], dB[8];
-            float um2[5], um1[5], u0[5], up1[5];
-            float s0, s1, s2, s3;
-            if (idx < NFT) { // tau face (f, j): cells f-1 (below) and f
-                const int rf = idx / TJ, c = idx - rf * TJ;
-                const int f = i0 + rf, j = j0 + c;
-                if (f > ni || j >= nj) continue;
-                const int R = rf + 3, Q = c + 3;
-                nx = C.tnx[cidx(f, j, pitch)];
-                ny = C.tny[cidx(f, j, pitch)];
+            // ---- tau face f = r-2 (between rows f-1 and f) ----
+            const int f = r - 2;
+            float GT[5];
+            if (ST || (f >= is && f <= ie))
+                face_flux(k, U[0], U[1], U[2], U[3], SI[0], SI[1], SI[2], SI[3], RT[1] + RT[0], D[0], D[1],
+                          TN[1][0], TN[1][1], GT);
+
+            // ---- cell row c = r-3: n faces, residual, update ----
+            const int c = r - 3;
+            if (ST || (c >= is && c < ie)) {
+                float um2[5], um1[5], up1[5], dA[8];
 #pragma unroll
                 for (int m = 0; m < 5; ++m) {
-                    um2[m] = sm.a.u[m][R - 2][Q];
-                    um1[m] = sm.a.u[m][R - 1][Q];
-                    u0[m] = sm.a.u[m][R][Q];
-                    up1[m] = sm.a.u[m][R + 1][Q];
+                    um1[m] = from_left(U[1][m]);
+                    um2[m] = from_left(um1[m]);
+                    up1[m] = from_right(U[1][m]);
                 }
-                s0 = sm.si[rf][c];
-                s1 = sm.si[rf + 1][c];
-                s2 = sm.si[rf + 2][c];
-                s3 = sm.si[rf + 3][c];
-                lam = sm.ratio[rf + 1][c + 1] + sm.ratio[rf][c + 1];
 #pragma unroll
-                for (int m = 0; m < 8; ++m) {
-                    dA[m] = sm.d[m][rf][c + 1];
-                    dB[m] = sm.d[m][rf + 1][c + 1];
-                }
-            } else { // n face (i, g): cells g-1 (left) and g
-                const int id2 = idx - NFT;
-                const int r = id2 / (TJ + 1), cg = id2 - r * (TJ + 1);
-                const int i = i0 + r, gcol = j0 + cg;
-                if (i >= ni || gcol > nj) continue;
-                const int R = r + 3, Q = cg + 3;
-                const int gw = sm.col[Q];
-                nx = C.nnx[cidx(i, gw, pitch)];
-                ny = C.nny[cidx(i, gw, pitch)];
+                for (int m = 0; m < 8; ++m) dA[m] = from_left(D[0][m]);
+                const float s1 = from_left(SJ[0]);
+                const float s0 = from_left(s1);
+                const float s3 = from_right(SJ[0]);
+                float GL[5], GR[5], R5[5];
+                face_flux(k, um2, um1, U[1], up1, s0, s1, SJ[0], s3, RT[0] + from_left(RT[0]), dA, D[0],
+                          NN[0][0], NN[0][1], GL);
 #pragma unroll
                 for (int m = 0; m < 5; ++m) {
-                    um2[m] = sm.a.u[m][R][Q - 2];
-                    um1[m] = sm.a.u[m][R][Q - 1];
-                    u0[m] = sm.a.u[m][R][Q];
-                    up1[m] = sm.a.u[m][R][Q + 1];
+                    GR[m] = from_right(GL[m]);
+                    R5[m] = GT[m] - GTp[m] + GR[m] - GL[m];
                 }
-                s0 = sm.sj[r][cg];
-                s1 = sm.sj[r][cg + 1];
-                s2 = sm.sj[r][cg + 2];
-                s3 = sm.sj[r][cg + 3];
-                lam = sm.ratio[r + 1][cg + 1] + sm.ratio[r + 1][cg];
+                const int q = cidx(c, jw, pitch);
+                if (MODE == MIX_TERMS) {
+                    if (own) {
 #pragma unroll
-                for (int m = 0; m < 8; ++m) {
-                    dA[m] = sm.d[m][r + 1][cg];
-                    dB[m] = sm.d[m][r + 1][cg + 1];
+                        for (int m = 0; m < 5; ++m) C.dbg[m * plane + q] = R5[m];
+                        C.dbg[5 * plane + q] = S[0];
+                    }
+                } else if (MODE == MIX_INNER) {
+                    R5[4] -= S[0];
+                    const float sc = (float)(k.lts ? alpha * (double)ldt_c : a_dt) * IV[0];
+                    float nw[5];
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) nw[m] = base[m] - R5[m] * sc;
+                    float p_, u_, v_, t_, n_;
+                    prim_from_du(k, nw, p_, u_, v_, t_, n_);
+                    const float rho = k.rho_inf + nw[0], pr = k.p_inf + p_;
+                    if (own) {
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) sdst[m * plane + q] = nw[m];
+                        // NonPhysical (state.rs:100-117), also catching NaN
+                        if (!(rho > 1e-15f) || !(pr > 1e-15f))
+                            atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
+                                                         (unsigned long long)((long long)c * nj + jg));
+                    }
+                } else { // MIX_LAST
+                    R5[4] -= S[0];
+                    const double sc = (k.lts ? alpha * (double)ldt_c : a_dt) * (double)IV[0];
+                    double u1[5];
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) u1[m] = former[m] - (double)R5[m] * sc;
+                    // NonPhysical test without a division: rho > 0 there, so
+                    // p <= 1e-15 <=> p * rho <= 1e-15 * rho
+                    const double prho = (k.k.gamma - 1.0) * (u1[3] * u1[0] - 0.5 * (u1[1] * u1[1] + u1[2] * u1[2]));
+                    const bool bad = isnan(u1[0]) || u1[0] <= 1e-15 || isnan(prho) || prho <= 1e-15 * u1[0];
+                    // next step's local dt and V/dt (timestep.rs:24-47), FP32
+                    const float rho = (float)u1[0], inv = frcp(rho);
+                    const float vx = (float)u1[1] * inv, vy = (float)u1[2] * inv;
+                    const float pr = (float)prho * inv;
+                    const float aa = 0.5f * (TN[0][0] + TN[1][0]), bb = 0.5f * (TN[0][1] + TN[1][1]);
+                    const float cc = 0.5f * (NN[0][0] + from_right(NN[0][0]));
+                    const float dd = 0.5f * (NN[0][1] + from_right(NN[0][1]));
+                    const float cs = fsqrt(k.gamma * fmaxf(pr, 0.0f) * inv);
+                    const float conv = fabsf(vx * aa + vy * bb) + fabsf(vx * cc + vy * dd);
+                    const float acou = cs * (fsqrt(aa * aa + bb * bb) + fsqrt(cc * cc + dd * dd));
+                    const float ldt = k.cfl * VOL[0] / (conv + acou);
+                    const double dr = u1[0] - former[0];
+                    double d2 = own ? dr * dr : 0.0;
+                    double dm = own ? (double)ldt : __builtin_huge_val();
+                    if (own) {
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) {
+                            C.u64[rot ^ 1][m * plane + q] = u1[m];
+                            C.sb[m * plane + q] = (float)(u1[m] - k.u64[m]);
+                        }
+                        C.localdt[rot ^ 1][q] = ldt;
+                        C.ratio[rot ^ 1][q] = (conv + acou) * k.inv_cfl;
+                        if (bad)
+                            atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
+                                                         (unsigned long long)((long long)c * nj + jg));
+                    }
+                    d2 = seg8_sum(d2);
+                    dm = seg8_min(dm);
+                    if (own && ((lane - 4) & 7) == 0) {
+                        const int ch = c * C.nchunk + (jg >> 3);
+                        C.rpart[ch] = d2;
+                        C.dpart[ch] = dm;
+                    }
                 }
             }
-            gsel[q] = idx;
-            lam *= k.cfl_half;
-            const float mx = fmaxf(fmaxf(fmaxf(s0, s1), s2), s3);
-            e2 = k.k2 * mx;
-            e4 = fmaxf(k.k4 - e2, 0.0f);
-            float fc[5];
-            euler_flux_pert(k, um1, u0, nx, ny, fc);
-            const float hx = 0.5f * nx, hy = 0.5f * ny;
+
+            // ---- shift the windows ----
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
-                const float d1 = u0[m] - um1[m];
-                const float d3 = up1[m] - 3.0f * u0[m] + 3.0f * um1[m] - um2[m];
-                const float fd = (d1 * e2 - d3 * e4) * lam;
-                const float fv = m == 0 ? 0.0f : (dA[m - 1] + dB[m - 1]) * hx + (dA[m + 3] + dB[m + 3]) * hy;
-                G[q][m] = fc[m] - fv - fd;
+                GTp[m] = GT[m];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) U[q][m] = U[q + 1][m];
             }
-        }
-        __syncthreads(); // R3 arrays are dead from here on: the face arrays alias them
+            P[0] = P[1];
+            P[1] = P[2];
 #pragma unroll
-        for (int q = 0; q < FPT; ++q) {
-            const int idx = gsel[q];
-            if (idx < 0) continue;
-            if (idx < NFT) {
-                const int rf = idx / TJ, c = idx - rf * TJ;
-#pragma unroll
-                for (int m = 0; m < 5; ++m) sm.f.gt[m][rf][c] = G[q][m];
-            } else {
-                const int id2 = idx - NFT;
-                const int r = id2 / (TJ + 1), cg = id2 - r * (TJ + 1);
-#pragma unroll
-                for (int m = 0; m < 5; ++m) sm.f.gn[m][r][cg] = G[q][m];
+            for (int m = 0; m < 4; ++m) {
+                PR[0][m] = PR[1][m];
+                PR[1][m] = PR[2][m];
             }
-        }
-        __syncthreads();
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                SI[q] = SI[q + 1];
+                SJ[q] = SJ[q + 1];
+                RT[q] = RT[q + 1];
+                TN[q][0] = TN[q + 1][0];
+                TN[q][1] = TN[q + 1][1];
+            }
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+#pragma unroll
+                for (int m = 0; m < 8; ++m) D[q][m] = D[q + 1][m];
+                S[q] = S[q + 1];
+                NN[q][0] = NN[q + 1][0];
+                NN[q][1] = NN[q + 1][1];
+                IV[q] = IV[q + 1];
+                VOL[q] = VOL[q + 1];
+            }
+        };
 
-        // ---- phase 4: residual and FP64 update of the tile's own cells ----
-        for (int o = tid; o < TI * TJ; o += NT) { // uniform trip count (TI*TJ % NT == 0)
-            const int ti = o / TJ, tj = o - ti * TJ;
-            const int i = i0 + ti, j = j0 + tj;
-            const bool valid = i < ni && j < nj;
-            double d2 = 0.0, dmin = __builtin_huge_val();
-            if (valid) {
-                float R5[5];
-#pragma unroll
-                for (int m = 0; m < 5; ++m)
-                    R5[m] = sm.f.gt[m][ti + 1][tj] - sm.f.gt[m][ti][tj] + sm.f.gn[m][ti][tj + 1] - sm.f.gn[m][ti][tj];
-                const int q = cidx(i, j, pitch);
-                if (TERMS) { // export R = Fc - Fv - Fd per component and S separately
-#pragma unroll
-                    for (int m = 0; m < 5; ++m) C.dbg[m * plane + q] = R5[m];
-                    C.dbg[5 * plane + q] = sm.src[ti][tj];
-                } else {
-                    R5[4] -= sm.src[ti][tj];
-                    const double a_dt = k.lts ? alpha * (double)C.localdt[rot][q] : alpha * dt;
-                    const double sc = a_dt * (double)C.inv_vol[q];
-                    double un[5];
-#pragma unroll
-                    for (int m = 0; m < 5; ++m) {
-                        un[m] = former[m * plane + q] - (double)R5[m] * sc;
-                        sdst[m * plane + q] = (float)(un[m] - k.u64[m]);
-                    }
-                    // NonPhysical test (state.rs:100-117) without a division: rho > 0 there,
-                    // so p <= 1e-15 <=> p * rho <= 1e-15 * rho
-                    const double prho = (k.k.gamma - 1.0) * (un[3] * un[0] - 0.5 * (un[1] * un[1] + un[2] * un[2]));
-                    if (isnan(un[0]) || un[0] <= 1e-15 || isnan(prho) || prho <= 1e-15 * un[0])
-                        atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
-                                                     (unsigned long long)((long long)i * nj + j));
-                    if (last_stage) {
-                        const double inv = 1.0 / un[0];
-                        const double vx = un[1] * inv, vy = un[2] * inv;
-                        const double p = prho * inv;
-#pragma unroll
-                        for (int m = 0; m < 5; ++m) u_next[m * plane + q] = un[m];
-                        const double dr = un[0] - former[q];
-                        d2 = dr * dr;
-                        // next step's local dt and V/dt (jst-rs timestep.rs:24-47)
-                        const int jp = j + 1 < nj ? j + 1 : 0;
-                        const int qu = cidx(i + 1, j, pitch), qr = cidx(i, jp, pitch);
-                        const double aa = 0.5 * ((double)C.tnx[q] + (double)C.tnx[qu]);
-                        const double bb = 0.5 * ((double)C.tny[q] + (double)C.tny[qu]);
-                        const double cc = 0.5 * ((double)C.nnx[q] + (double)C.nnx[qr]);
-                        const double dd = 0.5 * ((double)C.nny[q] + (double)C.nny[qr]);
-                        const double cs = sqrt(k.k.gamma * fmax(p, 0.0) * inv);
-                        const double conv = fabs(vx * aa + vy * bb) + fabs(vx * cc + vy * dd);
-                        const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(cc * cc + dd * dd));
-                        const double vol = (double)C.vol[q];
-                        const double ldt = k.k.cfl * vol / (conv + acou);
-                        C.localdt[rot ^ 1][q] = (float)ldt;
-                        C.ratio[rot ^ 1][q] = (float)((conv + acou) * inv_cfl); // V / dt_local
-                        dmin = ldt;
-                    }
-                }
-            }
-            if (last_stage) {
-                // this row's 32-column chunk: fixed 32-lane butterflies (sum, min)
-                for (int off = 16; off > 0; off >>= 1) {
-                    d2 += __shfl_xor(d2, off, 32);
-                    dmin = fmin(dmin, __shfl_xor(dmin, off, 32));
-                }
-                if ((tid & 31) == 0 && i < ni && j < nj) {
-                    const int ch = i * C.nchunk + (j >> 5);
-                    C.rpart[ch] = d2;
-                    C.dpart[ch] = dmin;
-                }
-            }
-        }
-        __syncthreads(); // LDS is reused by the next tile of this block
+        using steady_t = std::integral_constant<bool, true>;
+        using edge_t = std::integral_constant<bool, false>;
+        // steady rows: r >= 2 and r >= is + 3 (cell row r-3 is owned, no ghost rows behind),
+        // r <= ie - 1 and r <= ni - 1 (the state of row r is interior)
+        const int rs0 = is + 3 > 2 ? is + 3 : 2, rs1 = ie - 1 < ni - 1 ? ie - 1 : ni - 1;
+        int r = is - 3;
+        load_bundle(r, edge_t{});
+        for (; r < rs0 && r <= ie + 2; ++r) row_step(r, edge_t{});
+        for (; r <= rs1; ++r) row_step(r, steady_t{});
+        for (; r <= ie + 2; ++r) row_step(r, edge_t{});
     }
 }
 
-// ---------------------------------------------------------------------------------------
-// dt refresh (case creation, residual-terms export): local dt and V/dt of the current state
-// into buffer [rot], global minimum into ctl->dt.  Work items are fixed 1024-cell chunks.
-// ---------------------------------------------------------------------------------------
 // ---------------------------------------------------------------------------------------
 // End of step, one block per case: residual sqrt(sum (rho^{n+1} - rho^n)^2 / NI / NJ)
 // (jst-rs solver.rs:190-200) and the next global dt from the per-(row, 32-column) partials in
@@ -717,39 +763,30 @@ __global__ void shadow_kernel(const CaseMix *__restrict__ cases, int ci)
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < 5 * plane;
          q += (size_t)gridDim.x * blockDim.x) {
         const int m = (int)(q / plane);
-        C.s32[rot][q] = (float)(C.u64[rot][q] - C.k.u64[m]);
+        C.sb[q] = (float)(C.u64[rot][q] - C.k.u64[m]);
     }
 }
 
 namespace jst {
 
-template <int TI, int TJ, int NT>
-static hipError_t stage_launch(const CaseMix *cases, const TileDesc *tiles, int ntiles, int grid,
-                               int stage, double alpha, bool terms, hipStream_t s)
+const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 48};
+
+hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
+                              int stage, double alpha, bool terms, hipStream_t s)
 {
-    if (grid <= 0 || grid > ntiles) grid = ntiles;
+    const int need = (nitems + 3) / 4; // four waves (work items) per workgroup
+    if (grid <= 0 || grid > need) grid = need;
+    if (grid <= 0) return hipSuccess;
     if (terms)
-        hipLaunchKernelGGL((mix_stage_kernel<TI, TJ, NT, true>), dim3(grid), dim3(NT), 0, s, cases,
-                           tiles, ntiles, stage, alpha);
+        hipLaunchKernelGGL((mix_march_kernel<MIX_TERMS>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
+                           stage, alpha);
+    else if (stage == 4)
+        hipLaunchKernelGGL((mix_march_kernel<MIX_LAST>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
+                           stage, alpha);
     else
-        hipLaunchKernelGGL((mix_stage_kernel<TI, TJ, NT, false>), dim3(grid), dim3(NT), 0, s, cases,
-                           tiles, ntiles, stage, alpha);
+        hipLaunchKernelGGL((mix_march_kernel<MIX_INNER>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
+                           stage, alpha);
     return hipGetLastError();
-}
-
-const MixVariant MIX_VARIANTS[MIX_NVARIANTS] = {{16, 32, 256}, {8, 64, 256}, {8, 32, 256},
-                                                 {16, 64, 512}};
-
-hipError_t launch_stage_mixed(int variant, const CaseMix *cases, const TileDesc *tiles, int ntiles,
-                              int grid, int stage, double alpha, bool terms, hipStream_t s)
-{
-    switch (variant) {
-    case 0: return stage_launch<16, 32, 256>(cases, tiles, ntiles, grid, stage, alpha, terms, s);
-    case 1: return stage_launch<8, 64, 256>(cases, tiles, ntiles, grid, stage, alpha, terms, s);
-    case 2: return stage_launch<8, 32, 256>(cases, tiles, ntiles, grid, stage, alpha, terms, s);
-    case 3: return stage_launch<16, 64, 512>(cases, tiles, ntiles, grid, stage, alpha, terms, s);
-    default: return hipErrorInvalidValue;
-    }
 }
 
 hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s)
