@@ -555,8 +555,9 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     if (hc->cfg.halo != JST_H) return fail(JST_ERR_CONFIG, "unsupported halo: kernels are built for IM=3");
     if (mesh.n_rings - 1 < JST_H || mesh.n_theta < 4)
         return fail(JST_ERR_ARG, "the mixed path needs at least 3 cell rings and 4 cells per ring");
-    if ((size_t)mesh.n_rings * ((mesh.n_theta + 31) / 32 * 32) * 6 >= ((size_t)1 << 31))
-        return fail(JST_ERR_ARG, "mesh too large for 32-bit plane offsets");
+    // every array of a case (at most 6 planes of (NI+1) x pitch, 8 bytes) below 4 GiB
+    if ((size_t)mesh.n_rings * ((mesh.n_theta + 31) / 32 * 32) * 6 * 8 >= ((size_t)1 << 32))
+        return fail(JST_ERR_ARG, "mesh too large for 32-bit array offsets");
     int slot = -1;
     for (int q = 0; q < b->max_cases; ++q)
         if (!b->cases[q]) {

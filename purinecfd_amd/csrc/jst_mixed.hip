@@ -26,7 +26,7 @@ __device__ __forceinline__ int wrapj(int j, int nj)
     return j < 0 ? j + nj : j;
 }
 // 32-bit cell offsets: the host rejects cases whose planes exceed 2^31 / 8 elements
-__device__ __forceinline__ int cidx(int i, int j, int pitch) { return i * pitch + j; }
+__device__ __forceinline__ unsigned cidx(int i, int j, int pitch) { return (unsigned)(i * pitch + j); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
@@ -273,6 +273,22 @@ __device__ __forceinline__ double seg8_min(double x)
 
 enum { MIX_INNER = 0, MIX_LAST = 1, MIX_TERMS = 2 };
 
+// Global-address-space views: loads and stores become global_* with a scalar base and a
+// 32-bit lane offset instead of flat accesses with per-lane 64-bit addresses.
+template <class T> using gptr = __attribute__((address_space(1))) T *;
+template <class T> __device__ __forceinline__ gptr<const T> G(const T *p) { return (gptr<const T>)p; }
+template <class T> __device__ __forceinline__ gptr<T> G(T *p) { return (gptr<T>)p; }
+// Element q of a plane with a 32-bit byte offset (the host keeps every case below 4 GiB per
+// array), so that the address is a scalar base plus a 32-bit lane offset.
+template <class T> __device__ __forceinline__ T ldg(const T *p, unsigned q)
+{
+    return *(gptr<const T>)((gptr<const char>)p + q * (unsigned)sizeof(T));
+}
+template <class T> __device__ __forceinline__ void stg(T *p, unsigned q, T v)
+{
+    *(gptr<T>)((gptr<char>)p + q * (unsigned)sizeof(T)) = v;
+}
+
 } // namespace
 
 // ---------------------------------------------------------------------------------------
@@ -298,13 +314,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 {
     const int lane = threadIdx.x & 63;
     const int nwaves = gridDim.x * 4;
-    for (int it = blockIdx.x * 4 + (threadIdx.x >> 6); it < nitems; it += nwaves) {
+    // the work item is wave-uniform: keep it (and everything derived from it) in SGPRs
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int it = blockIdx.x * 4 + wave; it < nitems; it += nwaves) {
         const TileDesc w = items[it];
         const CaseMix &C = cases[w.c];
         const MixConst &k = C.k;
         CaseCtl *ctl = C.ctl;
         const int ni = C.ni, nj = C.nj, pitch = C.pitch;
-        const int plane = ni * pitch;
+        const unsigned plane = (unsigned)(ni * pitch);
         const int is = w.i0, J0 = w.j0, ie = w.t;
         const int status = *(volatile int *)&ctl->status;
         const long long step = *(volatile long long *)&ctl->step;
@@ -325,6 +343,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         while (jw < 0) jw += nj;
         while (jw >= nj) jw -= nj;
         const bool own = lane >= 4 && lane < 60 && jg < nj;
+
+        // per-component plane bases (wave-uniform)
+        const float *srcm[5], *sbm[5];
+        float *dstm[5], *sbw[5];
+        const double *fm[5];
+        double *unm[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            srcm[m] = src + m * plane;
+            sbm[m] = C.sb + m * plane;
+            sbw[m] = C.sb + m * plane;
+            dstm[m] = sdst + m * plane;
+            fm[m] = C.u64[rot] + m * plane;
+            unm[m] = C.u64[rot ^ 1] + m * plane;
+        }
 
         // register windows (index 0 = oldest row)
         float U[5][5];     // state dU, rows r-4 .. r
@@ -350,36 +383,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             constexpr bool ST = decltype(steady)::value;
             if (ST || r < ni) {
                 const int rr = ST ? (r < ni ? r : ni - 1) : (r >= 0 ? r : -1 - r); // -1-r: wall mirror
-                const int q = cidx(rr, jw, pitch);
+                const unsigned q = cidx(rr, jw, pitch);
 #pragma unroll
-                for (int m = 0; m < 5; ++m) bu[m] = src[m * plane + q];
+                for (int m = 0; m < 5; ++m) bu[m] = ldg(srcm[m], q);
             }
             const int rc = r < 0 ? 0 : (r > ni - 1 ? ni - 1 : r);
-            brt = ratio_now[cidx(rc, jw, pitch)];
+            brt = ldg(ratio_now, cidx(rc, jw, pitch));
             if (ST || (r >= 0 && r <= ni)) {
-                btn[0] = C.tnx[cidx(r, jw, pitch)];
-                btn[1] = C.tny[cidx(r, jw, pitch)];
+                btn[0] = ldg(C.tnx, cidx(r, jw, pitch));
+                btn[1] = ldg(C.tny, cidx(r, jw, pitch));
             }
             const int x = r - 1;
             if (ST || (x >= 0 && x < ni)) {
-                const int q = cidx(x, jw, pitch);
-                bnn[0] = C.nnx[q];
-                bnn[1] = C.nny[q];
-                biv = C.inv_vol[q];
-                bvol = C.vol[q];
-                biw = C.inv_wd2[q];
+                const unsigned q = cidx(x, jw, pitch);
+                bnn[0] = ldg(C.nnx, q);
+                bnn[1] = ldg(C.nny, q);
+                biv = ldg(C.inv_vol, q);
+                bvol = ldg(C.vol, q);
+                biw = ldg(C.inv_wd2, q);
             }
             const int c = r - 3;
             if (MODE != MIX_TERMS && (ST || (c >= is && c < ie))) {
-                const int q = cidx(c, jw, pitch);
+                const unsigned q = cidx(c, jw, pitch);
                 if (MODE == MIX_INNER) {
 #pragma unroll
-                    for (int m = 0; m < 5; ++m) bsb[m] = C.sb[m * plane + q];
+                    for (int m = 0; m < 5; ++m) bsb[m] = ldg(sbm[m], q);
                 } else {
 #pragma unroll
-                    for (int m = 0; m < 5; ++m) bf[m] = C.u64[rot][m * plane + q];
+                    for (int m = 0; m < 5; ++m) bf[m] = ldg(fm[m], q);
                 }
-                if (k.lts) bldt = C.localdt[rot][q];
+                if (k.lts) bldt = ldg(C.localdt[rot], q);
             }
         };
 
@@ -513,12 +546,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     GR[m] = from_right(GL[m]);
                     R5[m] = GT[m] - GTp[m] + GR[m] - GL[m];
                 }
-                const int q = cidx(c, jw, pitch);
+                const unsigned q = cidx(c, jw, pitch);
                 if (MODE == MIX_TERMS) {
                     if (own) {
 #pragma unroll
-                        for (int m = 0; m < 5; ++m) C.dbg[m * plane + q] = R5[m];
-                        C.dbg[5 * plane + q] = S[0];
+                        for (int m = 0; m < 5; ++m) stg(C.dbg + m * plane, q, R5[m]);
+                        stg(C.dbg + 5 * plane, q, S[0]);
                     }
                 } else if (MODE == MIX_INNER) {
                     R5[4] -= S[0];
@@ -531,7 +564,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     const float rho = k.rho_inf + nw[0], pr = k.p_inf + p_;
                     if (own) {
 #pragma unroll
-                        for (int m = 0; m < 5; ++m) sdst[m * plane + q] = nw[m];
+                        for (int m = 0; m < 5; ++m) stg(dstm[m], q, nw[m]);
                         // NonPhysical (state.rs:100-117), also catching NaN
                         if (!(rho > 1e-15f) || !(pr > 1e-15f))
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
@@ -564,11 +597,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     if (own) {
 #pragma unroll
                         for (int m = 0; m < 5; ++m) {
-                            C.u64[rot ^ 1][m * plane + q] = u1[m];
-                            C.sb[m * plane + q] = (float)(u1[m] - k.u64[m]);
+                            stg(unm[m], q, u1[m]);
+                            stg(sbw[m], q, (float)(u1[m] - k.u64[m]));
                         }
-                        C.localdt[rot ^ 1][q] = ldt;
-                        C.ratio[rot ^ 1][q] = (conv + acou) * k.inv_cfl;
+                        stg(C.localdt[rot ^ 1], q, ldt);
+                        stg(C.ratio[rot ^ 1], q, (conv + acou) * k.inv_cfl);
                         if (bad)
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
                                                          (unsigned long long)((long long)c * nj + jg));
@@ -576,9 +609,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     d2 = seg8_sum(d2);
                     dm = seg8_min(dm);
                     if (own && ((lane - 4) & 7) == 0) {
-                        const int ch = c * C.nchunk + (jg >> 3);
-                        C.rpart[ch] = d2;
-                        C.dpart[ch] = dm;
+                        const unsigned ch = (unsigned)(c * C.nchunk + (jg >> 3));
+                        stg(C.rpart, ch, d2);
+                        stg(C.dpart, ch, dm);
                     }
                 }
             }
