@@ -3,7 +3,8 @@
 
 Default workload (the metric's "FP32-batched vs FP64-plain"): the 64-case uneven C4 batch
 (NACA 4-digit O-meshes 129x65 .. 513x257 drawn from seed 20261203).  A "step" is one RK step
-(5 stage launches + 1 step-end launch) of every case.  Reported:
+(5 stage launches + 1 step-end launch) of every case.  Tuning: the row-segment length
+and grid of the stage kernel, picked by the autotuner.  Reported:
 
 * value: FP32-mixed batched cell-stage updates/s = sum(NI*NJ)*5*steps / wall seconds, all
   64 cases resident in HBM and advanced by the same launches (HIP-graph replays), tile shape
@@ -14,8 +15,9 @@ Default workload (the metric's "FP32-batched vs FP64-plain"): the 64-case uneven
   residual evaluation by the test suite);
 * serving: p50/p99 time to completion of a Poisson request stream (mean 20 solves/s) run by
   the serving loop (jst_serve);
-* roofline: the fused FP32 stage kernel, 152 algorithmic B/cell-stage x cells / its mean
-  launch duration (HIP events around every stage launch on the batch's stream);
+* roofline: the dominant kernel, the FP32 row-march stage of RK stages 1-4 (4 of the 5
+  launches per step): 92 algorithmic B/cell x cells / its mean launch duration (HIP events
+  around every launch on the batch's stream); stage 5 and the step end are reported beside;
 * cpu_baseline: the oracle (C restatement of jst-rs, OpenMP) timed on this host's cores on a
   bounded sample (one 257x129 case of the batch).
 
@@ -35,7 +37,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-BYTES = {"fp64": 184.0, "mixed": 152.0}
+# Algorithmic HBM bytes per cell of one launch (each array read or written once):
+#   fp64 stage: U^(k-1) 40 + U^n 40 + U^(k) 40 + FP64 geometry 56 + frozen V/dt 8
+#   mixed RK stages 1-4: dU^(k-1) 20 + dU^n 20 + dU^(k) 20 + FP32 geometry 28 + V/dt 4
+#   mixed stage 5: dU^(4) 20 + U^n 40 + U^{n+1} 40 + dU^{n+1} 20 + geometry 28 + V/dt 4
+#                  + next dt and V/dt 8
+BYTES = {"fp64": 184.0, "mixed_inner": 92.0, "mixed_last": 160.0}
 METRIC = "cell-stage updates/s FP32-batched vs FP64-plain (ΔCL,ΔCD≤1e-4); p99 solve latency"
 
 
@@ -127,16 +134,21 @@ def bench_c4(args, jst):
     b = jst.Batch(hbm_bytes=8 << 30, max_cases=len(cases))
     for w in cases:
         b.add(w["config"], w["mesh"], 10**9)
-    variant, grid = b.tune(3)
+    if args.tiling:
+        variant, grid = (int(x) for x in args.tiling.split(","))
+        b.set_tiling(variant, grid)
+    else:
+        variant, grid = b.tune(3)
     if args.warmup > 0:
         b.time_steps(args.warmup)
     t0 = time.perf_counter()
     ev_ms, _ = b.time_steps(args.steps)
     wall = time.perf_counter() - t0
     value = total * 5 * args.steps / wall
-    _, stage_ms = b.time_steps(max(5, min(50, args.steps // 4)), per_stage=True)
+    kern = b.time_kernels(max(5, min(50, args.steps // 4)))
     b.close()
-    achieved = BYTES["mixed"] * total / (stage_ms * 1e-3) / 1e9
+    achieved = BYTES["mixed_inner"] * total / (kern["inner_ms"] * 1e-3) / 1e9
+    last_gbs = BYTES["mixed_last"] * total / (kern["last_ms"] * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": value, "unit": "cell-stage updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
@@ -144,7 +156,8 @@ def bench_c4(args, jst):
         "data": "synthetic: seeded NACA 4-digit O-meshes, freestream start",
         "config": {"workload": "C4 batch: 64 uneven NACA 4-digit O-mesh cases 129x65..513x257 "
                                "(seed 20261203), FP32-mixed batched, FP64 accumulation",
-                   "cases": len(cases), "cells": total, "tile_variant": variant, "grid": grid,
+                   "cases": len(cases), "cells": total, "segment_rows": [16, 24, 32, 48][variant],
+                   "grid": grid,
                    "event_ms_per_step": ev_ms / args.steps},
         "fp64_plain": {"value": plain_rate, "unit": "cell-stage updates/s",
                        "how": f"same 64 cases, FP64 plain path, one case at a time, "
@@ -155,9 +168,13 @@ def bench_c4(args, jst):
                      "reference": "FP64 plain GPU path (oracle-pinned) at the same step"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic("mix_stage", "c4_batch64"),
-                     "kernel": "mix_stage_kernel (FP32-mixed, all 64 cases per launch)",
-                     "stage_kernel_ms": stage_ms, "bytes_per_cell_stage": BYTES["mixed"]},
+                     "traffic": profiled_traffic("mix_march_inner", "c4_batch64"),
+                     "kernel": "mix_march_kernel<MIX_INNER> (RK stages 1-4, all 64 cases per launch)",
+                     "kernel_ms": kern["inner_ms"], "bytes_per_cell": BYTES["mixed_inner"],
+                     "stage5": {"kernel_ms": kern["last_ms"], "bytes_per_cell": BYTES["mixed_last"],
+                                "achieved_gbs": last_gbs,
+                                "traffic": profiled_traffic("mix_march_last", "c4_batch64")},
+                     "step_end_ms": kern["end_ms"]},
     }
     if args.throughput_only:
         return out
@@ -183,8 +200,9 @@ def bench_single(args, jst):
     ev_ms, _ = s.time_steps(args.steps)
     wall = time.perf_counter() - t0
     _, stage_ms = s.time_steps(max(10, min(100, args.steps // 4)), per_stage=True)
-    key = "mixed" if args.mixed else "fp64"
-    achieved = BYTES[key] * cells / (stage_ms * 1e-3) / 1e9
+    # mixed: mean over the 5 stage launches, so the byte figure is the same mean
+    nbytes = (4 * BYTES["mixed_inner"] + BYTES["mixed_last"]) / 5 if args.mixed else BYTES["fp64"]
+    achieved = nbytes * cells / (stage_ms * 1e-3) / 1e9
     out = {"metric": METRIC, "value": cells * 5 * args.steps / wall, "unit": "cell-stage updates/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -194,8 +212,8 @@ def bench_single(args, jst):
                       "cells": cells, "event_ms_per_step": ev_ms / args.steps},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS,
-                        "traffic": profiled_traffic("mix_stage" if args.mixed else "stage_fp64", w["name"]),
-                        "stage_kernel_ms": stage_ms, "bytes_per_cell_stage": BYTES[key]}}
+                        "traffic": None if args.mixed else profiled_traffic("stage_fp64", w["name"]),
+                        "stage_kernel_ms": stage_ms, "bytes_per_cell_stage": nbytes}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w["config"], w["mesh"], w["name"])
     return out
@@ -215,6 +233,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--throughput-only", action="store_true",
                     help="C4: only the batched throughput pass (for profiler runs)")
+    ap.add_argument("--tiling", default="",
+                    help="C4: 'variant,grid' of the stage kernel instead of autotuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the path shards by case (or by "

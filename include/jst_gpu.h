@@ -192,15 +192,22 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out);
 /* Residual history of a case: res[k] for steps first..first+n-1 (1-based), within the last
  * 4096 steps. */
 jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, double *res);
-/* Tile shape (variant index) and grid (0 = one workgroup per tile) of the stage kernel. */
+/* Work decomposition of the stage kernel: variant = row-segment length index (16, 24, 32 or
+ * 48 rows per wave work item of 56 columns), grid = workgroups of 4 waves (0 = one per 4
+ * work items).  The grid never changes results; the segment length changes them only at
+ * FP32 rounding level (where a segment's edge rows are evaluated). */
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);
-/* Autotuner: times every tile variant and grid size on the current batch for trial_steps
- * steps each (the steps are real work), keeps the fastest; returns the choice. */
+/* Autotuner: times every variant and grid size on the current batch for trial_steps steps
+ * each (the steps are real work), keeps the fastest; returns the choice. */
 jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant, int *grid);
 /* Benchmark hook: nsteps steps of every running case with convergence stops disabled;
  * total_ms from HIP events around the region, stage_ms (may be NULL) the mean duration of
  * one stage launch from a second pass with events around every stage launch. */
 jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, double *stage_ms);
+/* Benchmark hook: nsteps steps with HIP events around every launch; mean durations of the
+ * RK stage 1-4 launches (inner_ms), the stage-5 launch (last_ms) and the step end (end_ms). */
+jst_status jst_batch_time_kernels(jst_batch *b, int nsteps, double *inner_ms, double *last_ms,
+                                  double *end_ms);
 jst_status jst_batch_counts(const jst_batch *b, int *active_cases, long *active_cells);
 /* Profiling calibration (no reference counterpart): one device-to-device copy of `bytes`
  * bytes, so that profiler byte counters can be scaled against a transfer of known size. */

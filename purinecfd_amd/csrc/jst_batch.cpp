@@ -908,6 +908,53 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
     return JST_OK;
 }
 
+jst_status jst_batch_time_kernels(jst_batch *b, int nsteps, double *inner_ms, double *last_ms,
+                                  double *end_ms)
+{
+    if (!b || nsteps <= 0) return fail(JST_ERR_ARG, "bad arguments");
+    jst_status st = fetch_ctl(b);
+    if (st != JST_OK) return st;
+    std::vector<int> touched;
+    for (auto &hc : b->cases)
+        if (hc && b->h_ctl[hc->slot].status == JST_RUN_ACTIVE) {
+            if ((st = set_ctl_field(b, hc->slot, &CaseCtl::timing, 1)) != JST_OK) return st;
+            touched.push_back(hc->slot);
+        }
+    if (b->dirty && (st = rebuild_lists(b)) != JST_OK) return st;
+    std::vector<hipEvent_t> evs(12 * (size_t)nsteps);
+    for (auto &e : evs) HIPCHK(hipEventCreate(&e));
+    for (int s = 0; s < nsteps; ++s) {
+        for (int k = 0; k < 5; ++k) {
+            HIPCHK(hipEventRecord(evs[12 * s + 2 * k], b->stream));
+            HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
+                                           false, b->stream));
+            HIPCHK(hipEventRecord(evs[12 * s + 2 * k + 1], b->stream));
+        }
+        HIPCHK(hipEventRecord(evs[12 * s + 10], b->stream));
+        HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
+        HIPCHK(hipEventRecord(evs[12 * s + 11], b->stream));
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < nsteps; ++s)
+        for (int k = 0; k < 6; ++k) {
+            float e = 0.f;
+            HIPCHK(hipEventElapsedTime(&e, evs[12 * s + 2 * k], evs[12 * s + 2 * k + 1]));
+            acc[k < 4 ? 0 : (k == 4 ? 1 : 2)] += e;
+        }
+    for (auto &e : evs) (void)hipEventDestroy(e);
+    if (inner_ms) *inner_ms = acc[0] / (4.0 * nsteps);
+    if (last_ms) *last_ms = acc[1] / nsteps;
+    if (end_ms) *end_ms = acc[2] / nsteps;
+    for (int slot : touched)
+        if ((st = set_ctl_field(b, slot, &CaseCtl::timing, 0)) != JST_OK) return st;
+    if ((st = fetch_ctl(b)) != JST_OK) return st;
+    for (int slot : touched)
+        if (b->h_ctl[slot].err_key != JST_ERR_NONE)
+            return fail(JST_ERR_NONPHYSICAL, "non-physical state during timing");
+    return JST_OK;
+}
+
 // Calibration probe for the HBM byte counters: one device-to-device copy of `bytes`.
 jst_status jst_copy_probe(size_t bytes, double *ms)
 {
@@ -932,12 +979,13 @@ jst_status jst_copy_probe(size_t bytes, double *ms)
     return JST_OK;
 }
 
-jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant_out, int *grid_out)
+static jst_status tune_impl(jst_batch *b, int trial_steps, bool variants, int *variant_out, int *grid_out)
 {
     if (!b || trial_steps <= 0) return fail(JST_ERR_ARG, "bad arguments");
     int best_v = b->variant, best_g = b->grid;
     double best = 1e300;
-    for (int v = 0; v < MIX_NVARIANTS; ++v) {
+    const int v0 = variants ? 0 : b->variant, v1 = variants ? MIX_NVARIANTS : b->variant + 1;
+    for (int v = v0; v < v1; ++v) {
         // grid 0: one workgroup per tile; otherwise a persistent grid of k workgroups per CU
         for (int g : {0, 256 * 2, 256 * 4}) {
             jst_status st = jst_batch_set_tiling(b, v, g);
@@ -956,6 +1004,11 @@ jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant_out, int *
     if (variant_out) *variant_out = best_v;
     if (grid_out) *grid_out = best_g;
     return JST_OK;
+}
+
+jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant_out, int *grid_out)
+{
+    return tune_impl(b, trial_steps, true, variant_out, grid_out);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1015,7 +1068,9 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
         }
         if (opts->tune && admitted && active != tuned_for && active >= 4) {
             int v, g;
-            if ((st = jst_batch_tune(b, 3, &v, &g)) != JST_OK) return st;
+            // grid only: the segment length stays fixed, so a request's result does not
+            // depend on which other requests share the batch
+            if ((st = tune_impl(b, 3, false, &v, &g)) != JST_OK) return st;
             tuned_for = active;
         }
         if ((st = jst_batch_step(b, spr, 1)) != JST_OK) return st;

@@ -681,9 +681,11 @@ __global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__
     if (*(volatile int *)&ctl->status != JST_RUN_ACTIVE) return;
     const int n = C.ni * C.nchunk;
     double s = 0.0, m = __builtin_huge_val();
+    // sequential per lane (fixed order); unrolled so that the loads are in flight together
+#pragma unroll 16
     for (int t = tid; t < n; t += NT) {
-        s += C.rpart[t];
-        m = fmin(m, C.dpart[t]);
+        s += ldg(C.rpart, (unsigned)t);
+        m = fmin(m, ldg(C.dpart, (unsigned)t));
     }
     for (int off = 32; off > 0; off >>= 1) {
         s += __shfl_xor(s, off, 64);

@@ -35,7 +35,8 @@ EXPORTS = ["jst_last_error", "jst_config_check", "jst_mesh_parse", "jst_create",
            "jst_batch_create", "jst_batch_destroy", "jst_case_footprint", "jst_batch_add",
            "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces",
            "jst_batch_get_field", "jst_batch_residual_terms", "jst_batch_history",
-           "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_steps", "jst_batch_counts",
+           "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_steps", "jst_batch_time_kernels",
+           "jst_batch_counts",
            "jst_copy_probe", "jst_serve"]
 JST_ERR_NOSPACE = 11
 CASE_RUNNING, CASE_CONVERGED, CASE_STOPPED, CASE_NONPHYSICAL = 0, 1, 2, 3
@@ -150,6 +151,7 @@ def lib():
         L.jst_batch_set_tiling.argtypes = [vp, ip, ip]
         L.jst_batch_tune.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
         L.jst_batch_time_steps.argtypes = [vp, ip, dp, dp]
+        L.jst_batch_time_kernels.argtypes = [vp, ip, dp, dp, dp]
         L.jst_batch_counts.argtypes = [vp, C.POINTER(ip), C.POINTER(C.c_long)]
         L.jst_copy_probe.argtypes = [sz, dp]
         L.jst_serve.argtypes = [C.POINTER(_Request), ip, C.POINTER(_ServeOpts),
@@ -443,6 +445,12 @@ class Batch:
         _check(lib().jst_batch_time_steps(self._h, int(nsteps), C.byref(total),
                                           C.byref(stage) if per_stage else None))
         return total.value, (stage.value if per_stage else None)
+
+    def time_kernels(self, nsteps: int):
+        """Mean launch durations (ms) of RK stages 1-4, stage 5 and the step end."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().jst_batch_time_kernels(self._h, int(nsteps), C.byref(a), C.byref(b), C.byref(c)))
+        return {"inner_ms": a.value, "last_ms": b.value, "end_ms": c.value}
 
     def counts(self):
         n, cells = C.c_int(), C.c_long()

@@ -116,7 +116,12 @@ def c5_requests(seconds: float = 60.0, rate: float = 20.0, seed: int = 20261204,
     """Serving stream: Poisson arrivals (mean `rate` solves/s) over `seconds`; each request is
     a cylinder case (40x17, 80x33 or 160x65 points, M 0.2, Re 1e3, CFL 2, converged at a
     5-decade residual drop, at most 4000 steps) with probability `cylinder_fraction`, else a
-    draw from the C4 distribution (at most 2000 steps, 6-decade drop)."""
+    draw from the C4 distribution (at most 800 steps, 6-decade drop).
+
+    The C4 budget stays below the first divergence seen in that distribution: with the
+    reference's global time step (convective spectral radius only) about half of its cases
+    become non-physical somewhere between ~900 and 6000 steps, on the FP64 path as on the
+    mixed one."""
     rng = random.Random(seed)
     pool = c4_batch(n=64, seed=seed)
     out, t = [], 0.0
@@ -134,5 +139,5 @@ def c5_requests(seconds: float = 60.0, rate: float = 20.0, seed: int = 20261204,
         else:
             w = pool[rng.randrange(len(pool))]
             out.append({"arrival_s": t, "config": w["config"], "mesh": w["mesh"],
-                        "max_steps": 2000, "name": w["name"]})
+                        "max_steps": 800, "name": w["name"]})
     return out
