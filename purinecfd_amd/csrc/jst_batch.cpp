@@ -105,8 +105,8 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
-    L.rpart = carve((size_t)ni * ((nj + 7) / 8) * 8);
-    L.dpart = carve((size_t)ni * ((nj + 7) / 8) * 8);
+    L.rpart = carve((size_t)((ni + 7) / 8) * nj * 8);
+    L.dpart = carve((size_t)((ni + 7) / 8) * nj * 8);
     L.res_hist = carve((size_t)res_cap * 8);
     L.posts = carve((size_t)npost * sizeof(PostDesc));
     L.scratch = carve((2 * (size_t)nj + 2) * 8);
@@ -588,7 +588,6 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     cm.nj = hc->nj;
     cm.pitch = hc->pitch;
     cm.npost = hc->npost;
-    cm.nchunk = (hc->nj + 7) / 8; // every chunk holds at least one column: all are written
     cm.sb = F(L.sb);
     for (int q = 0; q < 2; ++q) {
         cm.u64[q] = D(L.u64[q]);

@@ -94,7 +94,6 @@ struct MixConst {
 struct CaseMix {
     int ni, nj, pitch;
     int npost;                    // dt-refresh blocks of this case (fixed 1024-cell chunks)
-    int nchunk;                   // 8-column chunks per row, ceil(nj / 8)
     double *u64[2];               // U^n / U^{n+1}, 5 planes each
     float *sb;                    // FP32 base dU^n = U^n - U_inf, 5 planes
     float *s32[2];                // FP32 stage states dU^(k), 5 planes each (ping-pong)
@@ -103,8 +102,8 @@ struct CaseMix {
     const float *vol, *inv_vol, *inv_wd2;
     const double *tnx64, *tny64, *tmx64, *tmy64, *cx64, *cy64; // far-field support (FP64)
     float *ratio[2], *localdt[2]; // frozen per step; [rot] is read by the step, [rot^1] written
-    double *rpart;                // residual partial sums per (row, 8-column chunk)
-    double *dpart;                // next-dt minima per (row, 8-column chunk)
+    double *rpart;                // residual partial sums per (8-row block, column)
+    double *dpart;                // next-dt minima per (8-row block, column)
     double *res_hist;
     int res_cap;
     CaseCtl *ctl;

@@ -255,22 +255,6 @@ __device__ __forceinline__ void face_flux(const MixConst &k, const float um2[5],
     }
 }
 
-// Sum (or minimum) of the 8 lanes 4+8q .. 11+8q, left in lane 4+8q, in a fixed order.
-__device__ __forceinline__ double seg8_sum(double x)
-{
-    x += __shfl_down(x, 1, 64);
-    x += __shfl_down(x, 2, 64);
-    x += __shfl_down(x, 4, 64);
-    return x;
-}
-__device__ __forceinline__ double seg8_min(double x)
-{
-    x = fmin(x, __shfl_down(x, 1, 64));
-    x = fmin(x, __shfl_down(x, 2, 64));
-    x = fmin(x, __shfl_down(x, 4, 64));
-    return x;
-}
-
 enum { MIX_INNER = 0, MIX_LAST = 1, MIX_TERMS = 2 };
 
 // Global-address-space views: loads and stores become global_* with a scalar base and a
@@ -304,7 +288,8 @@ template <class T> __device__ __forceinline__ void stg(T *p, unsigned q, T v)
 // Every face is evaluated once; the right n face of a lane is its neighbour's left face.
 // MIX_INNER (stages 1-4): U^(k) = U^n - a_k dt/V R in FP32 from the FP32 base dU^n.
 // MIX_LAST (stage 5): FP64 U^{n+1} = U^n - dt/V R, the new FP32 base, the next frozen dt
-// and V/dt, and the step-end partials per (row, 8-column chunk).
+// and V/dt, and the step-end partials per (8-row block, column), summed down each column in
+// row order, so they do not depend on the segment length or the batch.
 // MIX_TERMS: export R = Fc - Fv - Fd and S of the current state (no update).
 // ---------------------------------------------------------------------------------------
 template <int MODE>
@@ -372,10 +357,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         float NN[3][2];    // n-face normals (left face of the lane's cell), rows r-3 .. r-1
         float IV[3], VOL[3]; // 1/V and V, rows r-3 .. r-1
         float GTp[5];      // tau face r-3 (previous step)
+        float farc[10] = {}; // far-field ghost state (dU, dp, du, dv, dT, nu~)
+        double racc = 0.0;       // stage 5: sum of (rho^{n+1} - rho^n)^2 over this 8-row block
+        float dacc = __builtin_huge_valf(); // stage 5: minimum next local dt over the block
         // bundle of row r, loaded during step r-1
         float bu[5] = {}, brt = 0.f, btn[2] = {}, bnn[2] = {}, biv = 0.f, bvol = 0.f, biw = 0.f;
         float bsb[5] = {}, bldt = 0.f;
-        double bf[5] = {};
 
         // Loads of row r's bundle.  In a steady step every condition below holds except that
         // row r may be the first far-field ghost row: its (unused) state load is clamped.
@@ -408,9 +395,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                 if (MODE == MIX_INNER) {
 #pragma unroll
                     for (int m = 0; m < 5; ++m) bsb[m] = ldg(sbm[m], q);
-                } else {
-#pragma unroll
-                    for (int m = 0; m < 5; ++m) bf[m] = ldg(fm[m], q);
                 }
                 if (k.lts) bldt = ldg(C.localdt[rot], q);
             }
@@ -430,9 +414,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             for (int m = 0; m < 5; ++m) {
                 un[m] = bu[m];
                 base[m] = bsb[m];
-                former[m] = bf[m];
             }
             if (ST || r < ie + 2) load_bundle(r + 1, steady);
+            // stage 5: U^n of cell row r-3, loaded within the step (keeps 10 registers out of
+            // the prefetch bundle); the row's work below covers the latency
+            if (MODE == MIX_LAST && (ST || (r - 3 >= is && r - 3 < ie))) {
+                const unsigned q = cidx(r - 3, jw, pitch);
+#pragma unroll
+                for (int m = 0; m < 5; ++m) former[m] = ldg(fm[m], q);
+            }
 
             // ---- row r: state and primitives, ghosts synthesised ----
             float dp, du, dv, dT, nut;
@@ -446,20 +436,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     dv = -dv - 2.0f * k.v_inf;
                     nut = -nut;
                 }
-            } else { // far-field ghost rows ni .. ni+2 (boundary.rs:112-185)
-                double fs[7];
-                far_state64(C, src, jw, fs);
-                const double *Ui = C.k.u64;
-                un[0] = (float)(fs[0] - Ui[0]);
-                un[1] = (float)(fs[0] * fs[3] - Ui[1]);
-                un[2] = (float)(fs[0] * fs[4] - Ui[2]);
-                un[3] = (float)(fs[0] * fs[5] - Ui[3]);
-                un[4] = (float)(fs[0] * fs[6] - Ui[4]);
-                dp = (float)(fs[1] - C.k.k.p_inf);
-                du = (float)(fs[3] - C.k.k.u_inf);
-                dv = (float)(fs[4] - C.k.k.v_inf);
-                dT = (float)(fs[2] - C.k.k.t_inf);
-                nut = (float)fs[6];
+            } else {
+                // far-field ghost rows ni .. ni+2 (boundary.rs:112-185): one state for all
+                // three, formed when the march reaches row ni (always the first of them)
+                if (r == ni) {
+                    double fs[7];
+                    far_state64(C, src, jw, fs);
+                    const double *Ui = C.k.u64;
+                    farc[0] = (float)(fs[0] - Ui[0]);
+                    farc[1] = (float)(fs[0] * fs[3] - Ui[1]);
+                    farc[2] = (float)(fs[0] * fs[4] - Ui[2]);
+                    farc[3] = (float)(fs[0] * fs[5] - Ui[3]);
+                    farc[4] = (float)(fs[0] * fs[6] - Ui[4]);
+                    farc[5] = (float)(fs[1] - C.k.k.p_inf);
+                    farc[6] = (float)(fs[3] - C.k.k.u_inf);
+                    farc[7] = (float)(fs[4] - C.k.k.v_inf);
+                    farc[8] = (float)(fs[2] - C.k.k.t_inf);
+                    farc[9] = (float)fs[6];
+                }
+#pragma unroll
+                for (int m = 0; m < 5; ++m) un[m] = farc[m];
+                dp = farc[5];
+                du = farc[6];
+                dv = farc[7];
+                dT = farc[8];
+                nut = farc[9];
             }
 #pragma unroll
             for (int m = 0; m < 5; ++m) U[4][m] = un[m];
@@ -559,14 +560,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     float nw[5];
 #pragma unroll
                     for (int m = 0; m < 5; ++m) nw[m] = base[m] - R5[m] * sc;
-                    float p_, u_, v_, t_, n_;
-                    prim_from_du(k, nw, p_, u_, v_, t_, n_);
-                    const float rho = k.rho_inf + nw[0], pr = k.p_inf + p_;
+                    // p * rho = (gamma - 1) (E rho - |m|^2 / 2): the positivity test of p needs no
+                    // division while rho > 0
+                    const float rho = k.rho_inf + nw[0], mx = k.m_inf + nw[1], my = k.n_inf + nw[2];
+                    const float prho = k.gm1 * ((k.e_inf + nw[3]) * rho - 0.5f * (mx * mx + my * my));
                     if (own) {
 #pragma unroll
                         for (int m = 0; m < 5; ++m) stg(dstm[m], q, nw[m]);
                         // NonPhysical (state.rs:100-117), also catching NaN
-                        if (!(rho > 1e-15f) || !(pr > 1e-15f))
+                        if (!(rho > 1e-15f) || !(prho > 1e-15f * rho))
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
                                                          (unsigned long long)((long long)c * nj + jg));
                     }
@@ -592,8 +594,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     const float acou = cs * (fsqrt(aa * aa + bb * bb) + fsqrt(cc * cc + dd * dd));
                     const float ldt = k.cfl * VOL[0] / (conv + acou);
                     const double dr = u1[0] - former[0];
-                    double d2 = own ? dr * dr : 0.0;
-                    double dm = own ? (double)ldt : __builtin_huge_val();
+                    // this column's residual sum and dt minimum over an 8-row block
+                    racc += own ? dr * dr : 0.0;
+                    dacc = own ? fminf(dacc, ldt) : dacc;
                     if (own) {
 #pragma unroll
                         for (int m = 0; m < 5; ++m) {
@@ -606,12 +609,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
                                                          (unsigned long long)((long long)c * nj + jg));
                     }
-                    d2 = seg8_sum(d2);
-                    dm = seg8_min(dm);
-                    if (own && ((lane - 4) & 7) == 0) {
-                        const unsigned ch = (unsigned)(c * C.nchunk + (jg >> 3));
-                        stg(C.rpart, ch, d2);
-                        stg(C.dpart, ch, dm);
+                    if ((c & 7) == 7 || c == ie - 1) { // block end (segments start at multiples of 8)
+                        if (own) {
+                            const unsigned ch = (unsigned)((c >> 3) * nj + jg);
+                            stg(C.rpart, ch, racc);
+                            stg(C.dpart, ch, (double)dacc);
+                        }
+                        racc = 0.0;
+                        dacc = __builtin_huge_valf();
                     }
                 }
             }
@@ -665,7 +670,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 
 // ---------------------------------------------------------------------------------------
 // End of step, one block per case: residual sqrt(sum (rho^{n+1} - rho^n)^2 / NI / NJ)
-// (jst-rs solver.rs:190-200) and the next global dt from the per-(row, 32-column) partials in
+// (jst-rs solver.rs:190-200) and the next global dt from the per-(8-row block, column) partials in
 // a fixed order, totaltime += dt, step += 1, buffer rotation, and the stop decision
 // (convergence / relative drop / step limit, or the non-physical record).
 // ---------------------------------------------------------------------------------------
@@ -679,7 +684,7 @@ __global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__
     CaseCtl *ctl = C.ctl;
     const int tid = threadIdx.x;
     if (*(volatile int *)&ctl->status != JST_RUN_ACTIVE) return;
-    const int n = C.ni * C.nchunk;
+    const int n = ((C.ni + 7) >> 3) * C.nj;
     double s = 0.0, m = __builtin_huge_val();
     // sequential per lane (fixed order); unrolled so that the loads are in flight together
 #pragma unroll 16
@@ -804,7 +809,7 @@ __global__ void shadow_kernel(const CaseMix *__restrict__ cases, int ci)
 
 namespace jst {
 
-const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 48};
+const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64};
 
 hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
                               int stage, double alpha, bool terms, hipStream_t s)

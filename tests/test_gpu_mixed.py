@@ -116,7 +116,7 @@ def test_segment_length_and_grid(jst):
     rounding (relative 1e-6 after 40 steps), and bitwise for a fixed segment length."""
     cfg, text = _cases()[1]
     runs = {}
-    for variant, grid in ((0, 0), (0, 3), (1, 0), (1, 16), (2, 0), (3, 0), (3, 1024)):
+    for variant, grid in ((0, 0), (0, 3), (1, 0), (1, 16), (2, 0), (3, 0), (4, 0), (5, 0), (5, 1024)):
         b = jst.Batch(1 << 30, 2)
         b.set_tiling(variant, grid)
         cid = b.add(cfg, text, 40)
