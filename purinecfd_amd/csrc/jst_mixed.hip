@@ -309,18 +309,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         const int ni = C.ni, nj = C.nj, pitch = C.pitch;
         const unsigned plane = (unsigned)(ni * pitch);
         const int is = w.i0, J0 = w.j0, ie = w.t;
-        const int status = *(volatile int *)&ctl->status;
+        // control words: read once per item, made wave-uniform so that every pointer and
+        // branch derived from them lives in scalar registers
+        const int status = __builtin_amdgcn_readfirstlane(*(volatile int *)&ctl->status);
         const long long step = *(volatile long long *)&ctl->step;
         const unsigned long long ek = *(volatile unsigned long long *)&ctl->err_key;
-        const int rot = *(volatile int *)&ctl->rot;
+        const int rot = __builtin_amdgcn_readfirstlane(*(volatile int *)&ctl->rot);
         const long long seq = step * 5 + stage;
         if (MODE != MIX_TERMS) {
-            if (status != JST_RUN_ACTIVE) continue;                          // wave-uniform
-            if (ek != JST_ERR_NONE && (long long)(ek >> 32) < seq) continue; // failed earlier
+            if (status != JST_RUN_ACTIVE) continue; // wave-uniform
+            const int failed = __builtin_amdgcn_readfirstlane(
+                (int)(ek != JST_ERR_NONE && (long long)(ek >> 32) < seq));
+            if (failed) continue; // failed in an earlier stage
         }
         const float *src = (MODE == MIX_TERMS || stage == 0) ? C.sb : C.s32[(stage - 1) & 1];
         float *sdst = C.s32[stage & 1];
         const float *ratio_now = C.ratio[rot];
+        float *ratio_next = C.ratio[rot ^ 1];
+        const float *ldt_now = C.localdt[rot];
+        float *ldt_next = C.localdt[rot ^ 1];
         const double a_dt = alpha * ctl->dt;
 
         const int jg = J0 - 4 + lane; // unwrapped column of this lane
@@ -396,7 +403,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 #pragma unroll
                     for (int m = 0; m < 5; ++m) bsb[m] = ldg(sbm[m], q);
                 }
-                if (k.lts) bldt = ldg(C.localdt[rot], q);
+                if (k.lts) bldt = ldg(ldt_now, q);
             }
         };
 
@@ -415,14 +422,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                 un[m] = bu[m];
                 base[m] = bsb[m];
             }
-            if (ST || r < ie + 2) load_bundle(r + 1, steady);
             // stage 5: U^n of cell row r-3, loaded within the step (keeps 10 registers out of
-            // the prefetch bundle); the row's work below covers the latency
+            // the prefetch bundle) and before the prefetch, so that waiting for it (the
+            // memory counter retires in order) does not wait for the next row's bundle
             if (MODE == MIX_LAST && (ST || (r - 3 >= is && r - 3 < ie))) {
                 const unsigned q = cidx(r - 3, jw, pitch);
 #pragma unroll
                 for (int m = 0; m < 5; ++m) former[m] = ldg(fm[m], q);
             }
+            if (ST || r < ie + 2) load_bundle(r + 1, steady);
+            // keep the prefetch and U^n loads ahead of the row's arithmetic (the scheduler
+            // would otherwise sink some of them next to their use and wait on all loads there)
+            __builtin_amdgcn_sched_barrier(0);
 
             // ---- row r: state and primitives, ghosts synthesised ----
             float dp, du, dv, dT, nut;
@@ -603,8 +614,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                             stg(unm[m], q, u1[m]);
                             stg(sbw[m], q, (float)(u1[m] - k.u64[m]));
                         }
-                        stg(C.localdt[rot ^ 1], q, ldt);
-                        stg(C.ratio[rot ^ 1], q, (conv + acou) * k.inv_cfl);
+                        stg(ldt_next, q, ldt);
+                        stg(ratio_next, q, (conv + acou) * k.inv_cfl);
                         if (bad)
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
                                                          (unsigned long long)((long long)c * nj + jg));
