@@ -1,0 +1,77 @@
+"""jst-gpu, the command-line driver with jst-rs's CLI flags and output formats
+(jst-rs/src/bin/jst.rs:10-123), on the GPU."""
+from __future__ import annotations
+
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT, config_dict, mesh_text
+
+pytestmark = pytest.mark.gpu
+
+CLI = ROOT / "purinecfd_amd" / "_build" / "jst-gpu"
+RUST_EXP6 = r"-?\d\.\d{6}e-?\d+"
+
+
+@pytest.fixture(scope="module")
+def jst():
+    from purinecfd_amd import jst as mod
+    mod.lib()
+    return mod
+
+
+def run_cli(tmp_path, *args):
+    p = subprocess.run([str(CLI), *args], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    return p.returncode, p.stdout, p.stderr
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cli_outputs_match_the_reference_formats(tmp_path, oracle_lib, precision):
+    rc, out, err = run_cli(tmp_path, "--mesh", str(GOLDEN / "fangdata.txt"), "--config",
+                           str(GOLDEN / "config.json"), "--steps", "5", "--precision", precision)
+    assert rc == 0, err
+    lines = out.splitlines()
+    assert lines[0] == f"mesh {GOLDEN / 'fangdata.txt'}: 10 rings x 12 points -> 108 cells"
+    assert re.fullmatch(r"threads: \d+", lines[1])
+    steps = [ln for ln in lines if ln.startswith("step:")]
+    assert len(steps) == 5
+    for k, ln in enumerate(steps, 1):
+        assert re.fullmatch(rf"step:{k:6d} \| residual:{RUST_EXP6}", ln), ln
+    assert re.fullmatch(rf"steps: 5  final residual: {RUST_EXP6}", lines[-4])
+    assert re.fullmatch(rf"totaltime \(physical\): {RUST_EXP6} s", lines[-3])
+    assert re.fullmatch(r"wall clock: \d+\.\d{6} s  \(\d+\.\d{4} ms/step, \d+\.\d ns/cell/step\)", lines[-2])
+    assert lines[-1] == "all data is written in result.csv"
+    log = (tmp_path / "res.log").read_text().splitlines()
+    assert log[0] == "step,residual" and len(log) == 6
+    res = np.array([float(x.split(",")[1]) for x in log[1:]])
+    assert [int(x.split(",")[0]) for x in log[1:]] == [1, 2, 3, 4, 5]
+    o = oracle_lib.OracleSolver(config_dict(), mesh_text("fangdata.txt"))
+    want = np.array(o.run(5)["history"])
+    tol = 5e-7 if precision == "fp64" else 1e-4  # res.log keeps 7 significant digits
+    assert np.all(np.abs(res - want) <= tol * np.abs(want)), (res, want)
+    csv = (tmp_path / "result.csv").read_text().splitlines()
+    assert csv[0] == "i,j,x,y,rho,p,T,u,v,miubl" and len(csv) == 1 + 108
+
+
+def test_cli_csv_equals_the_solver_output(tmp_path, jst):
+    rc, _, err = run_cli(tmp_path, "--mesh", str(GOLDEN / "fangdata.txt"), "--config",
+                         str(GOLDEN / "config.json"), "--steps", "7", "--quiet")
+    assert rc == 0, err
+    s = jst.Solver((GOLDEN / "config.json").read_text(), mesh_text("fangdata.txt"))
+    s.run(7)
+    s.write_csv(tmp_path / "api.csv")
+    assert (tmp_path / "result.csv").read_text() == (tmp_path / "api.csv").read_text()
+
+
+def test_cli_reports_nonphysical(tmp_path):
+    cfg = config_dict()
+    cfg["simulation"]["CFL"] = 200.0
+    (tmp_path / "bad.json").write_text(__import__("json").dumps(cfg))
+    rc, out, err = run_cli(tmp_path, "--mesh", str(GOLDEN / "fangdata.txt"), "--config", "bad.json",
+                           "--steps", "50", "--quiet", "--no-output")
+    assert rc == 1
+    assert re.match(r"Error: NonPhysical \{ i: \d+, j: \d+, rho: ", err), err
+    assert not (tmp_path / "result.csv").exists()

@@ -141,3 +141,20 @@ def test_repository_mesh(jst):
     m = jst.Mesh.parse(mesh_text("fangdata.txt"))
     assert (m.n_rings, m.n_theta, m.n_cells()) == (10, 12, 108)
     assert abs(m.node(0, 0)[0] - 1.0) < 1e-12
+
+
+def test_cli_argument_and_input_errors(tmp_path):
+    """jst-gpu rejects unknown flags (exit 2) and reports unreadable inputs before touching
+    a GPU (exit 1, "Error: ..."), like the reference CLI's clap / io errors."""
+    import subprocess
+    from conftest import GOLDEN, ROOT
+    cli = ROOT / "purinecfd_amd" / "_build" / "jst-gpu"
+    p = subprocess.run([str(cli), "--bogus"], capture_output=True, text=True, cwd=tmp_path)
+    assert p.returncode == 2 and "unexpected argument --bogus" in p.stderr
+    p = subprocess.run([str(cli), "--mesh", str(tmp_path / "missing.txt"), "--config",
+                        str(GOLDEN / "config.json")], capture_output=True, text=True, cwd=tmp_path)
+    assert p.returncode == 1 and p.stderr.startswith("Error: ") and "missing.txt" in p.stderr
+    (tmp_path / "bad.txt").write_text("3 x\n")
+    p = subprocess.run([str(cli), "--mesh", str(tmp_path / "bad.txt"), "--config",
+                        str(GOLDEN / "config.json")], capture_output=True, text=True, cwd=tmp_path)
+    assert p.returncode == 1 and p.stderr.startswith("Error: ")
