@@ -242,14 +242,16 @@ __device__ __forceinline__ void face_flux(const MixConst &k, const float um2[5],
     const float mx = fmaxf(fmaxf(fmaxf(s0, s1), s2), s3);
     const float e2 = k.k2 * mx;
     const float e4 = fmaxf(k.k4 - e2, 0.0f);
+    const float le2 = lam * e2, le4 = lam * e4;
     float fc[5];
     euler_flux_pert(k, um1, u0, nx, ny, fc);
     const float hx = 0.5f * nx, hy = 0.5f * ny;
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
+        // d3 = up1 - 3 u0 + 3 um1 - um2 = (up1 - um2) - 3 d1
         const float d1 = u0[m] - um1[m];
-        const float d3 = up1[m] - 3.0f * u0[m] + 3.0f * um1[m] - um2[m];
-        const float fd = (d1 * e2 - d3 * e4) * lam;
+        const float d3 = (up1[m] - um2[m]) - 3.0f * d1;
+        const float fd = d1 * le2 - d3 * le4;
         const float fv = m == 0 ? 0.0f : (dA[m - 1] + dB[m - 1]) * hx + (dA[m + 3] + dB[m + 3]) * hy;
         G[m] = fc[m] - fv - fd;
     }
@@ -381,14 +383,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 #pragma unroll
                 for (int m = 0; m < 5; ++m) bu[m] = ldg(srcm[m], q);
             }
-            const int rc = r < 0 ? 0 : (r > ni - 1 ? ni - 1 : r);
-            brt = ldg(ratio_now, cidx(rc, jw, pitch));
-            if (ST || (r >= 0 && r <= ni)) {
+            // only what the segment's faces and cells use: V/dt of rows is-1 .. ie, tau
+            // normals of faces is-1 .. ie+1, cell geometry of rows is-1 .. ie
+            if (ST || (r >= is - 1 && r <= ie)) {
+                const int rc = r < 0 ? 0 : (r > ni - 1 ? ni - 1 : r);
+                brt = ldg(ratio_now, cidx(rc, jw, pitch));
+            }
+            if (ST || (r >= 0 && r <= ni && r >= is - 1 && r <= ie + 1)) {
                 btn[0] = ldg(C.tnx, cidx(r, jw, pitch));
                 btn[1] = ldg(C.tny, cidx(r, jw, pitch));
             }
             const int x = r - 1;
-            if (ST || (x >= 0 && x < ni)) {
+            if (ST || (x >= 0 && x < ni && x >= is - 1 && x <= ie)) {
                 const unsigned q = cidx(x, jw, pitch);
                 bnn[0] = ldg(C.nnx, q);
                 bnn[1] = ldg(C.nny, q);
@@ -504,15 +510,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     const float upx = TN[3][0], upy = TN[3][1], dnx = TN[2][0], dny = TN[2][1];
                     const float lfx = NN[2][0], lfy = NN[2][1];
                     const float rtx = from_right(lfx), rty = from_right(lfy);
+                    // sum over faces of (c + nb)/2 n / V = (c sum(n) + sum(nb n)) / (2V): the
+                    // cell's own value enters once through the (vanishing) closure sum(n)
+                    const float clx = upx - dnx + rtx - lfx, cly = upy - dny + rty - lfy;
+                    const float hiv = 0.5f * IV[2];
 #pragma unroll
                     for (int f = 0; f < 4; ++f) {
-                        const float cc = PR[1][f];
-                        const float f_up = 0.5f * (cc + PR[2][f]);
-                        const float f_dn = 0.5f * (cc + PR[0][f]);
-                        const float f_lf = 0.5f * (cc + from_left(cc));
-                        const float f_rt = 0.5f * (cc + from_right(cc));
-                        g[2 * f] = (f_up * upx - f_dn * dnx + f_rt * rtx - f_lf * lfx) * IV[2];
-                        g[2 * f + 1] = (f_up * upy - f_dn * dny + f_rt * rty - f_lf * lfy) * IV[2];
+                        const float cc = PR[1][f], lf = from_left(cc), rt = from_right(cc);
+                        const float sx = PR[2][f] * upx - PR[0][f] * dnx + rt * rtx - lf * lfx;
+                        const float sy = PR[2][f] * upy - PR[0][f] * dny + rt * rty - lf * lfy;
+                        g[2 * f] = (cc * clx + sx) * hiv;
+                        g[2 * f + 1] = (cc * cly + sy) * hiv;
                     }
                 }
                 float chi, fv1;
