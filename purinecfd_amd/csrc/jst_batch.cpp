@@ -465,7 +465,9 @@ jst_status batch_run_case(jst_batch *b, int id, long limit, jst_step_cb cb, void
             if (out) *out = jst_report{done, c.residual, c.totaltime, 0};
             const long long cell = (long long)(c.err_key & 0xFFFFFFFFull);
             char buf[160];
-            snprintf(buf, sizeof buf, "non-physical state at cell (%lld, %lld)", cell / hc->nj, cell % hc->nj);
+            const long long seq = (long long)(c.err_key >> 32);
+            snprintf(buf, sizeof buf, "non-physical state at cell (%lld, %lld) in RK stage %lld of step %lld",
+                     cell / hc->nj, cell % hc->nj, seq % 5 + 1, seq / 5 + 1);
             set_ctl_field(b, id, &CaseCtl::timing, 0);
             return fail(JST_ERR_NONPHYSICAL, buf);
         }

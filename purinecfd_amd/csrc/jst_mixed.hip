@@ -187,6 +187,14 @@ __device__ __forceinline__ void diffusion_tensor32(const MixConst &k, float dT, 
     fv1_out = f1;
 }
 
+// fw(r) of jst-rs source.rs:30-37 in FP64, as the plain path evaluates it.
+__device__ __noinline__ double fw64(const PhysConst &k, double r)
+{
+    const double r2 = r * r, gg = r + k.cw2 * (r2 * r2 * r2 - r);
+    const double g2 = gg * gg;
+    return gg * cbrt(sqrt(k.one_plus_cw3_6 / (g2 * g2 * g2 + k.cw3_6)));
+}
+
 // jst-rs source.rs:45-88 in FP32 (x^(1/6) via exp2/log2).
 __device__ __forceinline__ float sa_source32(const MixConst &k, float drho, float drn, float chi,
                                              float fv1, const float g[8], float inv_d2, float vol)
@@ -201,11 +209,19 @@ __device__ __forceinline__ float sa_source32(const MixConst &k, float drho, floa
     const float st = fmaxf(s + nu * inv_kd2 * fv2, 1e-10f);
     const float prod = k.cb1 * (1.0f - ft2) * st * rnu;
     const float r = fminf(nu * frcp(st) * inv_kd2, k.rmax);
-    const float r2 = r * r, r6 = r2 * r2 * r2;
-    const float gg = r + k.cw2 * (r6 - r);
-    const float g2 = gg * gg, g6 = g2 * g2 * g2;
-    const float q = k.one_plus_cw3_6 * frcp(g6 + k.cw3_6);
-    const float fw = gg * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(q) * (1.0f / 6.0f));
+    float fw;
+    if (r >= -8.0f) {
+        // g^6 stays below 1e32 for r in [-8, rmax = 10]: FP32 is safe
+        const float r2 = r * r, r6 = r2 * r2 * r2;
+        const float gg = r + k.cw2 * (r6 - r);
+        const float g2 = gg * gg, g6 = g2 * g2 * g2;
+        const float q = k.one_plus_cw3_6 * frcp(g6 + k.cw3_6);
+        fw = gg * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(q) * (1.0f / 6.0f));
+    } else {
+        // negative nu~ under a clamped S~ gives a large negative r: r^6 and g^6 overflow
+        // FP32 (inf * 0), so this rare case takes the FP64 formula of the plain path
+        fw = (float)fw64(k.k, (double)r);
+    }
     const float destr = (k.cw1 * fw - k.cb1 * k.inv_kappa2 * ft2) * rho * (nu * nu * inv_d2);
     const float diff = k.cb2 * k.sigma * rho * (g[6] * g[6] + g[7] * g[7]);
     return (prod - destr + diff) * vol;

@@ -199,3 +199,20 @@ def test_admission_by_footprint(jst):
     b.run(steps_per_round=10)
     b.remove(a)
     b.add(cfg, text, 10)
+
+
+def test_naca_fine_wall_spacing_tracks_fp64(jst):
+    """NACA 0012 257x129 with a 1e-5 c first spacing (M 0.75, Re 6e6): the trailing-edge
+    wall cells turn nu~ negative in the first steps, which drives the SA r to large negative
+    values (the FP32 fw would overflow; the mixed path takes the FP64 formula there).  The
+    mixed path must track the FP64 plain path: 40 steps, |dCL|, |dCD| <= 1e-5, field L2 <= 1e-5."""
+    from purinecfd_amd import workloads
+    w = workloads.c2_naca0012()
+    s64 = jst.Solver(w["config"], w["mesh"], jst.FP64)
+    smx = jst.Solver(w["config"], w["mesh"], jst.MIXED)
+    assert s64.run(40).steps == 40
+    assert smx.run(40).steps == 40
+    f, g = smx.forces(), s64.forces()
+    assert abs(f["cl"] - g["cl"]) <= 1e-5 and abs(f["cd"] - g["cd"]) <= 1e-5, (f, g)
+    for n in ("rho", "p", "u", "v"):
+        assert rel_l2(smx.get(n), s64.get(n)) <= 1e-5, n
