@@ -193,7 +193,7 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out);
  * 4096 steps. */
 jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, double *res);
 /* Work decomposition of the stage kernel: variant = row-segment length index (16, 24, 32, 40,
- * 48 or 64 rows per wave work item of 56 columns), grid = workgroups of 4 waves (0 = one per 4
+ * 48 or 64 rows per wave work item of 58 columns), grid = workgroups of 4 waves (0 = one per 4
  * work items).  The grid never changes results; the segment length changes them only at
  * FP32 rounding level (where a segment's edge rows are evaluated). */
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);

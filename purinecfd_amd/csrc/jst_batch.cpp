@@ -242,7 +242,7 @@ jst_status drop_graph(jst_batch *b)
     return JST_OK;
 }
 
-// Work items of one case for the row march: 56-column groups x row segments of `rows` rows.
+// Work items of one case for the row march: 58-column groups x row segments of `rows` rows.
 void case_items(const HostCase &hc, int rows, std::vector<TileDesc> &out)
 {
     for (int j0 = 0; j0 < hc.nj; j0 += MIX_COLS)

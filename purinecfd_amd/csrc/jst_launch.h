@@ -27,10 +27,10 @@ hipError_t launch_post_fp64(const CaseDev *cases, const TileDesc *tiles, int nti
 hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, double *out,
                               hipStream_t s);
 
-// FP32-mixed row-march variants: rows per work item (one wave per 56 columns x that many
+// FP32-mixed row-march variants: rows per work item (one wave per 58 columns x that many
 // rows); the autotuner picks one per batch.  Results do not depend on the variant.
 #define MIX_NVARIANTS 6
-#define MIX_COLS 56
+#define MIX_COLS 58
 extern const int MIX_SEGMENT_ROWS[MIX_NVARIANTS];
 hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
                               int stage, double alpha, bool terms, hipStream_t s);

@@ -196,12 +196,12 @@ __device__ __noinline__ double fw64(const PhysConst &k, double r)
 }
 
 // jst-rs source.rs:45-88 in FP32 (x^(1/6) via exp2/log2).
-__device__ __forceinline__ float sa_source32(const MixConst &k, float drho, float drn, float chi,
-                                             float fv1, const float g[8], float inv_d2, float vol)
+__device__ __forceinline__ float sa_source32(const MixConst &k, float drho, float drn, float nu,
+                                             float chi, float fv1, const float g[8], float inv_d2,
+                                             float vol)
 {
     const float rho = k.rho_inf + drho;
     const float rnu = k.rnu_inf + drn;
-    const float nu = rnu * frcp(rho);
     const float inv_kd2 = inv_d2 * k.inv_kappa2;
     const float ft2 = k.ct3 * __expf(-k.ct4 * chi * chi);
     const float fv2 = 1.0f - chi * frcp(1.0f + chi * fv1);
@@ -233,8 +233,8 @@ __device__ __forceinline__ float sensor32(float pinf4, float dm, float d0, float
     return fabsf((dm - 2.0f * d0 + dp) * frcp(pinf4 + dm + 2.0f * d0 + dp));
 }
 
-// Lane l of a wave owns column J0 + l - 4 for 4 <= l < 60 (56 columns); lanes 0..3 and 60..63
-// carry the periodic halo.  Values of the j-neighbours move between lanes with DPP wave shifts,
+// Lane l of a wave owns column J0 + l - 3 for 3 <= l < 61 (58 columns); lanes 0..2 and 61..63
+// carry the periodic halo (the widest stencil, a face's sensors, reaches 3 columns out).  Values of the j-neighbours move between lanes with DPP wave shifts,
 // which read every lane, so they are only ever called from wave-uniform control flow.
 __device__ __forceinline__ float from_left(float x) // lane l receives lane l-1
 {
@@ -294,7 +294,7 @@ template <class T> __device__ __forceinline__ void stg(T *p, unsigned q, T v)
 } // namespace
 
 // ---------------------------------------------------------------------------------------
-// FP32-mixed RK stage as a row march.  One wave per work item (case, 56-column group, row
+// FP32-mixed RK stage as a row march.  One wave per work item (case, 58-column group, row
 // segment [is, ie)): lanes hold columns, the wave walks rows is-3 .. ie+2 keeping short
 // windows of rows in registers (state, pressure, primitives, sensors, diffusion tensors,
 // frozen V/dt), with the next row's loads issued one step ahead.  After loading row r it
@@ -348,11 +348,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         float *ldt_next = C.localdt[rot ^ 1];
         const double a_dt = alpha * ctl->dt;
 
-        const int jg = J0 - 4 + lane; // unwrapped column of this lane
+        const int jg = J0 - 3 + lane; // unwrapped column of this lane
         int jw = jg;
         while (jw < 0) jw += nj;
         while (jw >= nj) jw -= nj;
-        const bool own = lane >= 4 && lane < 60 && jg < nj;
+        const bool own = lane >= 3 && lane < 61 && jg < nj;
 
         // per-component plane bases (wave-uniform)
         const float *srcm[5], *sbm[5];
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                 float chi, fv1;
                 diffusion_tensor32(k, PR[1][2], U[3][4], PR[1][0], PR[1][1], g, D[2], chi, fv1);
                 if (ST || (x >= is && x < ie && x < ni))
-                    S[2] = sa_source32(k, U[3][0], U[3][4], chi, fv1, g, iw_new, VOL[2]);
+                    S[2] = sa_source32(k, U[3][0], U[3][4], PR[1][3], chi, fv1, g, iw_new, VOL[2]);
                 if (!ST && x == 0) {
                     // ghost row -1 takes the gradient of row 0 without its temperature part
                     g[4] = 0.0f;
