@@ -162,7 +162,7 @@ def bench_c4(args, jst):
         "data": "synthetic: seeded NACA 4-digit O-meshes, freestream start",
         "config": {"workload": "C4 batch: 64 uneven NACA 4-digit O-mesh cases 129x65..513x257 "
                                "(seed 20261203), FP32-mixed batched, FP64 accumulation",
-                   "cases": len(cases), "cells": total, "segment_rows": [16, 24, 32, 40, 48, 64][variant],
+                   "cases": len(cases), "cells": total, "segment_rows": [16, 24, 32, 40, 48, 64, 8][variant],
                    "grid": grid,
                    "event_ms_per_step": ev_ms / args.steps},
         "fp64_plain": {"value": plain_rate, "unit": "cell-stage updates/s",

@@ -1023,6 +1023,14 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     jst_status st = jst_batch_create(JST_MIXED, opts->hbm_bytes, opts->max_cases, &b);
     if (st != JST_OK) return st;
     std::unique_ptr<jst_batch> guard(b);
+    {
+        const int rows = opts->segment_rows > 0 ? opts->segment_rows : 8;
+        int v = -1;
+        for (int q = 0; q < MIX_NVARIANTS; ++q)
+            if (jst::MIX_SEGMENT_ROWS[q] == rows) v = q;
+        if (v < 0) return fail(JST_ERR_ARG, "unsupported segment_rows");
+        if ((st = jst_batch_set_tiling(b, v, 0)) != JST_OK) return st;
+    }
     const int spr = opts->steps_per_round > 0 ? opts->steps_per_round : 50;
     std::vector<int> order(n);
     for (int q = 0; q < n; ++q) {

@@ -29,7 +29,7 @@ hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, dou
 
 // FP32-mixed row-march variants: rows per work item (one wave per 58 columns x that many
 // rows); the autotuner picks one per batch.  Results do not depend on the variant.
-#define MIX_NVARIANTS 6
+#define MIX_NVARIANTS 7
 #define MIX_COLS 58
 extern const int MIX_SEGMENT_ROWS[MIX_NVARIANTS];
 hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,

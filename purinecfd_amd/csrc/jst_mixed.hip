@@ -844,7 +844,7 @@ __global__ void shadow_kernel(const CaseMix *__restrict__ cases, int ci)
 
 namespace jst {
 
-const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64};
+const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64, 8};
 
 hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
                               int stage, double alpha, bool terms, hipStream_t s)

@@ -95,7 +95,7 @@ class _RequestResult(C.Structure):
 
 class _ServeOpts(C.Structure):
     _fields_ = [("hbm_bytes", C.c_size_t), ("max_cases", C.c_int),
-                ("steps_per_round", C.c_int), ("tune", C.c_int)]
+                ("steps_per_round", C.c_int), ("tune", C.c_int), ("segment_rows", C.c_int)]
 
 
 class _ServeSummary(C.Structure):
@@ -459,7 +459,7 @@ class Batch:
 
 
 def serve(requests, hbm_bytes: int = 16 << 30, max_cases: int = 64, steps_per_round: int = 50,
-          tune: bool = False):
+          tune: bool = False, segment_rows: int = 0):
     """Serving loop over ``requests`` = [(arrival_s, config, mesh, max_steps), ...].
 
     Returns (per-request results, summary) from jst_serve."""
@@ -472,7 +472,8 @@ def serve(requests, hbm_bytes: int = 16 << 30, max_cases: int = 64, steps_per_ro
         reqs[q] = _Request(float(t), cb, mb, int(steps))
     res = (_RequestResult * max(n, 1))()
     summ = _ServeSummary()
-    opts = _ServeOpts(int(hbm_bytes), int(max_cases), int(steps_per_round), 1 if tune else 0)
+    opts = _ServeOpts(int(hbm_bytes), int(max_cases), int(steps_per_round), 1 if tune else 0,
+                      int(segment_rows))
     _check(lib().jst_serve(reqs, n, C.byref(opts), res, C.byref(summ)))
     out = [{"admit_s": r.admit_s, "done_s": r.done_s, "steps": r.steps, "status": r.status,
             "residual": r.residual, "cl": r.cl, "cd": r.cd} for r in res[:n]]
