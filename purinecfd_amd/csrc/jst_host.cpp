@@ -789,6 +789,8 @@ jst_status jst_create(const char *config_json, const char *mesh_text, jst_precis
         jst_status sm = jst_case_footprint(config_json, mesh_text, prec, &bytes);
         if (sm != JST_OK) return sm;
         if ((sm = jst_batch_create(JST_MIXED, bytes + (4u << 20), 1, &x->mix)) != JST_OK) return sm;
+        // one case alone: the shortest row segments (8 rows, variant 6) give it the most waves
+        if ((sm = jst_batch_set_tiling(x->mix, 6, 0)) != JST_OK) return sm;
         if ((sm = jst_batch_add(x->mix, config_json, mesh_text, 0, &x->mix_id)) != JST_OK) return sm;
         x->d = finish(x->cfg);
         x->ni = x->mesh.n_rings - 1;
