@@ -126,6 +126,20 @@ jst_status jst_set_state(jst_ctx *ctx, const double *u_planes, jst_nonphysical *
  * of attack over q_inf * L.  Summed in FP64 in j order. */
 jst_status jst_forces(jst_ctx *ctx, double *fx, double *fy, double *cl, double *cd);
 
+/* No counterpart in the reference (wall diagnostics): per wall face j = 0..NJ-1 (the tau
+ * faces (0, j), normal n into the fluid with |n| = edge length, unit tangent
+ * s = (-n_y, n_x)/|n| in the direction of increasing j), from the same FP64 face fluxes as
+ * jst_forces: x, y = face midpoint; Cp = (p_w - p_inf)/q_inf with
+ * p_w = (Fc_mx n_x + Fc_my n_y)/|n|^2 (the mirrored wall ghost cancels the face momentum,
+ * so the convective momentum flux is p_w n); Cf = (Fv[mx,my] . s)/(|n| q_inf), the
+ * tangential traction the fluid exerts on the body.  Arrays of NJ doubles; any may be
+ * NULL. */
+jst_status jst_wall(jst_ctx *ctx, double *x, double *y, double *cp, double *cf);
+
+/* No counterpart in the reference: jst_wall as text, columns j,x,y,cp,cf (j 1-based,
+ * {:.8e} like result.csv). */
+jst_status jst_write_wall_csv(jst_ctx *ctx, const char *path);
+
 /* Solver::write_csv (solver.rs:229-255): columns i,j,x,y,rho,p,T,u,v,miubl, 1-based,
  * {:.8e}. */
 jst_status jst_write_csv(jst_ctx *ctx, const char *path);
@@ -184,6 +198,8 @@ jst_status jst_batch_remove(jst_batch *b, int case_id);
 jst_status jst_batch_step(jst_batch *b, int nsteps, int sync);
 jst_status jst_batch_info(const jst_batch *b, int case_id, jst_case_info *out);
 jst_status jst_batch_forces(jst_batch *b, int case_id, double *cl, double *cd);
+/* jst_wall for one case of a batch (FP64 wall terms from the case's FP64 state). */
+jst_status jst_batch_wall(jst_batch *b, int case_id, double *x, double *y, double *cp, double *cf);
 /* Field export for one case (names as jst_get_field: state, geometry, and after
  * jst_batch_residual_terms the FP32 residual R0..R4 = Fc-Fv-Fd per component and S). */
 jst_status jst_batch_get_field(jst_batch *b, int case_id, const char *name, double *dst,

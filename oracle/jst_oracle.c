@@ -1186,6 +1186,36 @@ void jo_forces(jo_solver *s, double *out4)
     out4[3] = (fx * ca + fy * sa) / ql;
 }
 
+/* EXT (not in the reference): wall distribution over the wall tau faces (0, j), j in
+ * [0, NJ), from the same face fluxes as jo_forces.  n = tau normal of the face (into the
+ * fluid, |n| = edge length), s = (-n_y, n_x)/|n| its unit tangent in the direction of
+ * increasing j.  x, y = face midpoint; p_w = (Fc_x n_x + Fc_y n_y)/|n|^2, the pressure the
+ * face's convective momentum flux carries (the mirrored ghost cancels the face momentum,
+ * so Fc[mx,my] = p_w n); Cp = (p_w - p_inf)/q_inf; Cf = (Fv[mx,my] . s)/(|n| q_inf), the
+ * tangential traction the fluid exerts on the body. */
+void jo_wall(jo_solver *s, double *x, double *y, double *cp, double *cf)
+{
+    int j;
+    jo_boundary_apply(s);
+    jo_convection(s);
+    jo_gradient(s);
+    jo_viscous(s);
+    {
+        double q = 0.5 * s->d.rho_inf * (s->d.u_inf * s->d.u_inf + s->d.v_inf * s->d.v_inf);
+        for (j = 0; j < s->nj; ++j) {
+            const double *g = s->tau + 4 * FI(s, 0, j);
+            const double *f = s->tflux + 5 * FI(s, 0, j), *v = s->tdiff + 5 * FI(s, 0, j);
+            const double n2 = g[0] * g[0] + g[1] * g[1];
+            const double pw = (f[1] * g[0] + f[2] * g[1]) / n2;
+            const double tau = (v[1] * (-g[1]) + v[2] * g[0]) / n2;
+            x[j] = g[2];
+            y[j] = g[3];
+            cp[j] = (pw - s->cfg.p_inf) / q;
+            cf[j] = tau / q;
+        }
+    }
+}
+
 /* ---- field export ---- */
 static long copy_cells(const jo_solver *s, const double *f, int ncomp, int comp, double *dst)
 {

@@ -126,6 +126,10 @@ long jo_get(const jo_solver *s, const char *name, double *dst);
 int jo_set_state(jo_solver *s, const double *u_planes, double *err);
 /* EXT: wall forces -> {Fx, Fy, CL, CD} from the current state (runs boundary+fluxes) */
 void jo_forces(jo_solver *s, double *out4);
+/* EXT: wall distribution per wall face j (NJ values each): face midpoint x, y, pressure
+ * coefficient Cp and skin friction Cf along the unit tangent of increasing j (runs
+ * boundary+fluxes) */
+void jo_wall(jo_solver *s, double *x, double *y, double *cp, double *cf);
 /* number of OpenMP threads the oracle will use */
 int jo_threads(void);
 

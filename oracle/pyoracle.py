@@ -91,6 +91,7 @@ def lib():
         L.jo_set_state.argtypes = [vp, dp, dp]
         L.jo_set_state.restype = C.c_int
         L.jo_forces.argtypes = [vp, dp]
+        L.jo_wall.argtypes = [vp, dp, dp, dp, dp]
         L.jo_mesh_parse.argtypes = [C.c_char_p, C.POINTER(JoMesh), C.POINTER(C.c_long),
                                     C.POINTER(C.c_long)]
         L.jo_mesh_parse.restype = C.c_int
@@ -242,6 +243,12 @@ class OracleSolver:
         out = (C.c_double * 4)()
         lib().jo_forces(self._h, out)
         return {"fx": out[0], "fy": out[1], "cl": out[2], "cd": out[3]}
+
+    def wall(self) -> dict:
+        a = {k: np.zeros(self.nj) for k in ("x", "y", "cp", "cf")}
+        ptr = [a[k].ctypes.data_as(C.POINTER(C.c_double)) for k in ("x", "y", "cp", "cf")]
+        lib().jo_wall(self._h, *ptr)
+        return a
 
 
 def load_config(path: str | os.PathLike) -> dict:

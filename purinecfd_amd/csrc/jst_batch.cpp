@@ -109,7 +109,7 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     L.dpart = carve((size_t)((ni + 7) / 8) * nj * 8);
     L.res_hist = carve((size_t)res_cap * 8);
     L.posts = carve((size_t)npost * sizeof(PostDesc));
-    L.scratch = carve((2 * (size_t)nj + 2) * 8);
+    L.scratch = carve((4 * (size_t)nj + 2) * 8); // wall-force terms [4][nj] + 2
     L.dbg = carve(6 * plane * 4);
     L.total = o;
     return L;
@@ -390,12 +390,10 @@ jst_status download(jst_batch *b, const void *d, size_t bytes, void *h)
 
 namespace jsth {
 
-// Wall force {Fx, Fy, CL, CD} of one case: the FP64 wall-force kernel runs on an FP64 view
-// of the case (its U^n buffers and FP64 geometry): forces are an FP64 reduction.
-jst_status batch_forces(jst_batch *b, int case_id, double out[4])
+// The FP64 wall-force kernel on an FP64 view of one case (its U^n buffers and FP64
+// geometry): per-face wall terms into the case's scratch, {Fx, Fy} into d_scratch_out.
+jst_status launch_wall_terms(jst_batch *b, HostCase *hc, int case_id)
 {
-    HostCase *hc = get_case(b, case_id);
-    if (!hc) return fail(JST_ERR_ARG, "no such case");
     CaseDev v{};
     v.ni = hc->ni;
     v.nj = hc->nj;
@@ -417,8 +415,18 @@ jst_status batch_forces(jst_batch *b, int case_id, double out[4])
     v.k = hc->dev.k.k;
     HIPCHK(hipMemcpyAsync(b->d_view, &v, sizeof(CaseDev), hipMemcpyHostToDevice, b->stream));
     HIPCHK(jst::launch_forces_fp64(b->d_view, 0, hc->scratch, b->d_scratch_out, b->stream));
+    return JST_OK;
+}
+
+// Wall force {Fx, Fy, CL, CD} of one case: forces are an FP64 reduction.
+jst_status batch_forces(jst_batch *b, int case_id, double out[4])
+{
+    HostCase *hc = get_case(b, case_id);
+    if (!hc) return fail(JST_ERR_ARG, "no such case");
+    jst_status st = launch_wall_terms(b, hc, case_id);
+    if (st != JST_OK) return st;
     double f[2];
-    jst_status st = download(b, b->d_scratch_out, sizeof f, f);
+    st = download(b, b->d_scratch_out, sizeof f, f);
     if (st != JST_OK) return st;
     const Config &c = hc->cfg;
     const double aoa = c.aoa * (M_PI / 180.0), ca = std::cos(aoa), sa = std::sin(aoa);
@@ -429,6 +437,17 @@ jst_status batch_forces(jst_batch *b, int case_id, double out[4])
     out[2] = (-f[0] * sa + f[1] * ca) / ql;
     out[3] = (f[0] * ca + f[1] * sa) / ql;
     return JST_OK;
+}
+
+// Wall distribution of one case from the same FP64 wall terms as its forces.
+jst_status batch_wall(jst_batch *b, int case_id, double *x, double *y, double *cp, double *cf)
+{
+    HostCase *hc = get_case(b, case_id);
+    if (!hc) return fail(JST_ERR_ARG, "no such case");
+    jst_status st = launch_wall_terms(b, hc, case_id);
+    if (st != JST_OK) return st;
+    return wall_distribution(hc->scratch, hc->g64, hc->nj, hc->cfg, hc->d, b->stream, x, y, cp,
+                             cf);
 }
 
 // Solver::run (jst-rs solver.rs:203-226) for one case of a batch: the device stops by itself
@@ -736,6 +755,12 @@ jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, doub
     if (st != JST_OK) return st;
     for (long q = 0; q < n; ++q) res[q] = h[(size_t)((first - 1 + q) % b->res_cap)];
     return JST_OK;
+}
+
+jst_status jst_batch_wall(jst_batch *b, int case_id, double *x, double *y, double *cp, double *cf)
+{
+    if (!b) return fail(JST_ERR_ARG, "batch is NULL");
+    return batch_wall(b, case_id, x, y, cp, cf);
 }
 
 jst_status jst_batch_forces(jst_batch *b, int case_id, double *cl, double *cd)

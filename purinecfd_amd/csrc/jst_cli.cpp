@@ -1,6 +1,6 @@
 // jst-gpu — command-line driver with the flags and outputs of jst-rs's CLI
 // (jst-rs/src/bin/jst.rs:10-123): --mesh --config --steps --out --reslog --quiet --threads
-// --no-output, plus --precision fp64|mixed.  stdout lines, res.log and result.csv use the
+// --no-output, plus --precision fp64|mixed and --wall PATH (wall Cp/Cf table).  stdout lines, res.log and result.csv use the
 // same text formats (Rust `{:6}`, `{:.6e}`, `{:.8e}`), so scripts that read the CPU
 // solver's output read this one's.  Built on the public C ABI only (include/jst_gpu.h).
 #include <algorithm>
@@ -31,6 +31,7 @@ std::string rust_exp(double v, int prec)
 
 struct Cli {
     std::string mesh = "fangdata.txt", config = "config.json", out = "result.csv", reslog = "res.log";
+    std::string wall; // empty: no wall table
     long steps = 0;
     bool quiet = false, no_output = false;
     long threads = 0;
@@ -42,7 +43,8 @@ int usage(const char *msg)
     if (msg) fprintf(stderr, "error: %s\n", msg);
     fprintf(stderr,
             "Usage: jst-gpu [--mesh PATH] [--config PATH] [--steps N] [--out PATH] [--reslog PATH]\n"
-            "               [--quiet] [--threads N] [--no-output] [--precision fp64|mixed]\n");
+            "               [--quiet] [--threads N] [--no-output] [--precision fp64|mixed]\n"
+            "               [--wall PATH]\n");
     return 2;
 }
 
@@ -80,6 +82,7 @@ int main(int argc, char **argv)
         else if (s == "--quiet") cli.quiet = true;
         else if (s == "--threads") cli.threads = atol(val("--threads"));
         else if (s == "--no-output") cli.no_output = true;
+        else if (s == "--wall") cli.wall = val("--wall");
         else if (s == "--precision") {
             std::string p = val("--precision");
             if (p == "fp64") cli.prec = JST_FP64;
@@ -147,6 +150,14 @@ int main(int argc, char **argv)
             return 1;
         }
         printf("all data is written in %s\n", cli.out.c_str());
+    }
+    if (!cli.wall.empty()) {
+        if (jst_write_wall_csv(ctx, cli.wall.c_str()) != JST_OK) {
+            fprintf(stderr, "Error: %s\n", jst_last_error());
+            jst_destroy(ctx);
+            return 1;
+        }
+        printf("wall data is written in %s\n", cli.wall.c_str());
     }
     jst_destroy(ctx);
     return 0;

@@ -2,11 +2,14 @@
 // batched/serving driver: Config (jst-rs config.rs), Mesh (mesh.rs), error text.
 #pragma once
 
+#include <hip/hip_runtime_api.h>
+
 #include <string>
 #include <vector>
 
 #include "../../include/jst_gpu.h"
 #include "jst_dev.h"
+#include "jst_launch.h"
 
 namespace jsth {
 
@@ -42,8 +45,18 @@ Derived finish(const Config &c);
 PhysConst make_const(const Config &c, const Derived &d);
 std::string rust_exp(double v, int prec);
 
+// Wall distribution (jst_wall): per wall face j, Cp from the pressure the tau face (0, j)
+// carries in its convective momentum flux, Cf from the tangential part of its viscous
+// momentum flux.  d_terms is the wall-force kernel's scratch [4][nj] (Fc_x | Fc_y | Fv_x |
+// Fv_y, device); g the case's FP64 geometry (device; row 0 of tnx/tny/tmx/tmy = the first nj
+// entries).  Output arrays are host memory; any may be NULL.
+jst_status wall_distribution(const double *d_terms, const jst::GeomPtrs &g, int nj,
+                             const Config &c, const Derived &d, hipStream_t s, double *x,
+                             double *y, double *cp, double *cf);
+
 // single-case use of a batch (the FP32-mixed jst_ctx)
 jst_status batch_forces(jst_batch *b, int case_id, double out[4]);
+jst_status batch_wall(jst_batch *b, int case_id, double *x, double *y, double *cp, double *cf);
 jst_status batch_run_case(jst_batch *b, int id, long limit, jst_step_cb cb, void *user,
                           jst_report *out, bool no_stop);
 

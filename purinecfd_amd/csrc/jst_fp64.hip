@@ -769,8 +769,10 @@ __global__ __launch_bounds__(NT) void post_kernel(const CaseDev *__restrict__ ca
 }
 
 // ---------------------------------------------------------------------------------------
-// Wall forces: per wall face j, (Fc - Fv)_tau(0,j) from the current state, then a serial
-// FP64 sum in j order by one thread.  out = {Fx, Fy}.
+// Wall forces: per wall face j, the momentum components of Fc_tau(0,j) and Fv_tau(0,j) from
+// the current state into scratch[4][nj] (Fc_x | Fc_y | Fv_x | Fv_y; the wall distribution
+// reads them back), then a serial FP64 sum of -(Fc - Fv) in j order by one thread.
+// out = {Fx, Fy}.
 // ---------------------------------------------------------------------------------------
 __global__ void forces_kernel(const CaseDev *__restrict__ cases, int ci, double *scratch,
                               double *out)
@@ -828,15 +830,17 @@ __global__ void forces_kernel(const CaseDev *__restrict__ cases, int ci, double 
         diffusion_tensor(k, qg.t, ug[4], qg.vx, qg.vy, g, dg, chi, fv1);
         const double fv1x = (dg[0] + d0[0]) * (0.5 * nx) + (dg[4] + d0[4]) * (0.5 * ny);
         const double fv2y = (dg[1] + d0[1]) * (0.5 * nx) + (dg[5] + d0[5]) * (0.5 * ny);
-        scratch[2 * j] = fc[1] - fv1x;
-        scratch[2 * j + 1] = fc[2] - fv2y;
+        scratch[j] = fc[1];
+        scratch[nj + j] = fc[2];
+        scratch[2 * nj + j] = fv1x;
+        scratch[3 * nj + j] = fv2y;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         double fx = 0.0, fy = 0.0;
         for (int j = 0; j < nj; ++j) {
-            fx -= scratch[2 * j];
-            fy -= scratch[2 * j + 1];
+            fx -= scratch[j] - scratch[2 * nj + j];
+            fy -= scratch[nj + j] - scratch[3 * nj + j];
         }
         out[0] = fx;
         out[1] = fy;

@@ -606,6 +606,33 @@ PhysConst make_const(const Config &c, const Derived &d)
     k.lts = c.lts ? 1 : 0;
     return k;
 }
+
+jst_status wall_distribution(const double *d_terms, const jst::GeomPtrs &g, int nj,
+                             const Config &c, const Derived &d, hipStream_t s, double *x,
+                             double *y, double *cp, double *cf)
+{
+    const size_t n = (size_t)nj;
+    std::vector<double> t(8 * n);
+    HIPCHK(hipMemcpyAsync(t.data(), d_terms, 4 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+    const double *rows[4] = {g.tnx, g.tny, g.tmx, g.tmy}; // tau faces of row 0: first nj entries
+    for (int q = 0; q < 4; ++q)
+        HIPCHK(hipMemcpyAsync(t.data() + (4 + q) * n, rows[q], n * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const double q = 0.5 * d.rho_inf * (d.u_inf * d.u_inf + d.v_inf * d.v_inf);
+    for (size_t j = 0; j < n; ++j) {
+        const double fx = t[j], fy = t[n + j], vx = t[2 * n + j], vy = t[3 * n + j];
+        const double nx = t[4 * n + j], ny = t[5 * n + j];
+        const double n2 = nx * nx + ny * ny;
+        const double pw = (fx * nx + fy * ny) / n2;
+        const double tau = (vx * (-ny) + vy * nx) / n2;
+        if (x) x[j] = t[6 * n + j];
+        if (y) y[j] = t[7 * n + j];
+        if (cp) cp[j] = (pw - c.p_inf) / q;
+        if (cf) cf[j] = tau / q;
+    }
+    return JST_OK;
+}
 } // namespace jsth
 
 namespace {
@@ -837,10 +864,10 @@ jst_status jst_create(const char *config_json, const char *mesh_text, jst_precis
     ALLOC(c.ratio, c.plane);
     ALLOC(c.localdt, c.plane);
     ALLOC(c.res_hist, (size_t)c.res_cap);
-    ALLOC(c.scratch, 2 * (size_t)c.nj + 2);
+    ALLOC(c.scratch, 4 * (size_t)c.nj + 2); // wall-force terms [4][nj] + {Fx, Fy}
     auto tiles = make_tiles(0, c.ni, c.nj, FP64_TI, FP64_TJ);
     c.ntiles = (int)tiles.size();
-    ALLOC(c.partial, (size_t)c.ntiles);
+    ALLOC(c.partial, 2 * (size_t)c.ntiles); // per-tile residual sums | per-tile min dt
     ALLOC(c.ctl, 1);
     ALLOC(c.d_case, 1);
     ALLOC(c.d_tiles, tiles.size());
@@ -1194,7 +1221,7 @@ jst_status jst_forces(jst_ctx *x, double *fx, double *fy, double *cl, double *cd
         if (cd) *cd = f4[3];
         return JST_OK;
     }
-    double *dout = x->scratch + 2 * (size_t)x->nj;
+    double *dout = x->scratch + 4 * (size_t)x->nj;
     HIPCHK(jst::launch_forces_fp64(x->d_case, 0, x->scratch, dout, x->stream));
     double f[2];
     HIPCHK(hipMemcpyAsync(f, dout, sizeof f, hipMemcpyDeviceToHost, x->stream));
@@ -1207,6 +1234,32 @@ jst_status jst_forces(jst_ctx *x, double *fx, double *fy, double *cl, double *cd
     if (fy) *fy = f[1];
     if (cl) *cl = (-f[0] * sa + f[1] * ca) / ql;
     if (cd) *cd = (f[0] * ca + f[1] * sa) / ql;
+    return JST_OK;
+}
+
+jst_status jst_wall(jst_ctx *x, double *wx, double *wy, double *cp, double *cf)
+{
+    if (!x) return fail(JST_ERR_ARG, "ctx is NULL");
+    if (x->mix) return batch_wall(x->mix, x->mix_id, wx, wy, cp, cf);
+    double *dout = x->scratch + 4 * (size_t)x->nj;
+    HIPCHK(jst::launch_forces_fp64(x->d_case, 0, x->scratch, dout, x->stream));
+    return wall_distribution(x->scratch, x->g, x->nj, x->cfg, x->d, x->stream, wx, wy, cp, cf);
+}
+
+jst_status jst_write_wall_csv(jst_ctx *x, const char *path)
+{
+    if (!x || !path) return fail(JST_ERR_ARG, "NULL argument");
+    const size_t n = (size_t)x->nj;
+    std::vector<double> wx(n), wy(n), cp(n), cf(n);
+    jst_status st = jst_wall(x, wx.data(), wy.data(), cp.data(), cf.data());
+    if (st != JST_OK) return st;
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(JST_ERR_IO, std::string("I/O error: cannot create ") + path);
+    fprintf(f, "j,x,y,cp,cf\n");
+    for (size_t j = 0; j < n; ++j)
+        fprintf(f, "%zu,%s,%s,%s,%s\n", j + 1, rust_exp(wx[j], 8).c_str(), rust_exp(wy[j], 8).c_str(),
+                rust_exp(cp[j], 8).c_str(), rust_exp(cf[j], 8).c_str());
+    fclose(f);
     return JST_OK;
 }
 

@@ -31,10 +31,10 @@ MESH_ERRORS = {JST_ERR_IO: "Io", JST_ERR_BAD_HEADER: "BadHeader", JST_ERR_EMPTY:
 EXPORTS = ["jst_last_error", "jst_config_check", "jst_mesh_parse", "jst_create",
            "jst_create_from_paths", "jst_destroy", "jst_dims", "jst_advance", "jst_run",
            "jst_residual", "jst_residual_terms", "jst_get_field", "jst_set_state", "jst_forces",
-           "jst_write_csv", "jst_progress", "jst_time_steps",
+           "jst_wall", "jst_write_wall_csv", "jst_write_csv", "jst_progress", "jst_time_steps",
            "jst_batch_create", "jst_batch_destroy", "jst_case_footprint", "jst_batch_add",
            "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces",
-           "jst_batch_get_field", "jst_batch_residual_terms", "jst_batch_history",
+           "jst_batch_wall", "jst_batch_get_field", "jst_batch_residual_terms", "jst_batch_history",
            "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_steps", "jst_batch_time_kernels",
            "jst_batch_counts",
            "jst_copy_probe", "jst_serve"]
@@ -133,6 +133,8 @@ def lib():
         L.jst_set_state.argtypes = [vp, dp, C.POINTER(_NonPhys)]
         L.jst_forces.argtypes = [vp, dp, dp, dp, dp]
         L.jst_write_csv.argtypes = [vp, C.c_char_p]
+        L.jst_wall.argtypes = [vp, dp, dp, dp, dp]
+        L.jst_write_wall_csv.argtypes = [vp, C.c_char_p]
         L.jst_progress.argtypes = [vp, C.POINTER(C.c_long), dp]
         L.jst_time_steps.argtypes = [vp, ip, dp, dp]
         sz = C.c_size_t
@@ -145,6 +147,7 @@ def lib():
         L.jst_batch_step.argtypes = [vp, ip, ip]
         L.jst_batch_info.argtypes = [vp, ip, C.POINTER(_CaseInfo)]
         L.jst_batch_forces.argtypes = [vp, ip, dp, dp]
+        L.jst_batch_wall.argtypes = [vp, ip, dp, dp, dp, dp]
         L.jst_batch_get_field.argtypes = [vp, ip, C.c_char_p, dp, sz]
         L.jst_batch_residual_terms.argtypes = [vp, ip, dp]
         L.jst_batch_history.argtypes = [vp, ip, C.c_long, C.c_long, dp]
@@ -337,6 +340,15 @@ class Solver:
         _check(lib().jst_forces(self._h, *[C.byref(x) for x in v]))
         return dict(zip(("fx", "fy", "cl", "cd"), (x.value for x in v)))
 
+    def wall(self) -> dict:
+        """Wall distribution per wall face j: midpoint x, y, Cp and Cf (jst_wall)."""
+        a = {k: np.zeros(self._nj) for k in ("x", "y", "cp", "cf")}
+        _check(lib().jst_wall(self._h, *(_dp(a[k]) for k in ("x", "y", "cp", "cf"))))
+        return a
+
+    def write_wall_csv(self, path: str | os.PathLike) -> None:
+        _check(lib().jst_write_wall_csv(self._h, str(path).encode()))
+
     def write_csv(self, path: str | os.PathLike) -> None:
         _check(lib().jst_write_csv(self._h, str(path).encode()))
 
@@ -415,6 +427,12 @@ class Batch:
         cl, cd = C.c_double(), C.c_double()
         _check(lib().jst_batch_forces(self._h, cid, C.byref(cl), C.byref(cd)))
         return {"cl": cl.value, "cd": cd.value}
+
+    def wall(self, cid: int, nj: int) -> dict:
+        """jst_wall for case cid (nj = its wall face count)."""
+        a = {k: np.zeros(nj) for k in ("x", "y", "cp", "cf")}
+        _check(lib().jst_batch_wall(self._h, cid, *(_dp(a[k]) for k in ("x", "y", "cp", "cf"))))
+        return a
 
     def get(self, cid: int, name: str, ni: int, nj: int) -> np.ndarray:
         rows = ni + 1 if name.startswith("tau_") else ni

@@ -66,6 +66,30 @@ def test_cli_csv_equals_the_solver_output(tmp_path, jst):
     assert (tmp_path / "result.csv").read_text() == (tmp_path / "api.csv").read_text()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cli_wall_table_equals_the_solver_output(tmp_path, jst, precision):
+    rc, out, err = run_cli(tmp_path, "--mesh", str(GOLDEN / "fangdata.txt"), "--config",
+                           str(GOLDEN / "config.json"), "--steps", "7", "--quiet", "--wall",
+                           "wall.csv", "--precision", precision)
+    assert rc == 0, err
+    assert out.splitlines()[-1] == "wall data is written in wall.csv"
+    rows = (tmp_path / "wall.csv").read_text().splitlines()
+    assert rows[0] == "j,x,y,cp,cf" and len(rows) == 1 + 12
+    assert [int(r.split(",")[0]) for r in rows[1:]] == list(range(1, 13))
+    for r in rows[1:]:
+        for v in r.split(",")[1:]:
+            assert re.fullmatch(r"-?\d\.\d{8}e-?\d+", v), r
+    prec = jst.FP64 if precision == "fp64" else jst.MIXED
+    s = jst.Solver((GOLDEN / "config.json").read_text(), mesh_text("fangdata.txt"), prec)
+    s.run(7)
+    s.write_wall_csv(tmp_path / "api_wall.csv")
+    assert (tmp_path / "wall.csv").read_text() == (tmp_path / "api_wall.csv").read_text()
+    w = s.wall()
+    got = np.array([[float(v) for v in r.split(",")[1:]] for r in rows[1:]])
+    for c, k in enumerate(("x", "y", "cp", "cf")):
+        assert np.allclose(got[:, c], w[k], rtol=1e-8, atol=0.0), k
+
+
 def test_cli_reports_nonphysical(tmp_path):
     cfg = config_dict()
     cfg["simulation"]["CFL"] = 200.0

@@ -217,3 +217,7 @@ def test_naca_fine_wall_spacing_tracks_fp64(jst):
     assert abs(f["cl"] - g["cl"]) <= 1e-5 and abs(f["cd"] - g["cd"]) <= 1e-5, (f, g)
     for n in ("rho", "p", "u", "v"):
         assert rel_l2(smx.get(n), s64.get(n)) <= 1e-5, n
+    wm, w64 = smx.wall(), s64.wall()
+    assert np.array_equal(wm["x"], w64["x"]) and np.array_equal(wm["y"], w64["y"])
+    assert np.max(np.abs(wm["cp"] - w64["cp"])) <= 1e-4
+    assert np.max(np.abs(wm["cf"] - w64["cf"])) <= 1e-4 * max(1.0, np.max(np.abs(w64["cf"])))

@@ -130,6 +130,9 @@ def test_cylinder_c1_matches_oracle(jst, oracle_lib):
     assert abs(f["cd"] - of["cd"]) <= 1e-8 * max(1.0, abs(of["cd"]))
     assert abs(f["cl"] - of["cl"]) <= 1e-8
     assert 0.5 < f["cd"] < 2.0  # a bluff body at Re 1e3: drag coefficient of order one
+    wg, wo = s.wall(), o.wall()
+    for k in ("x", "y", "cp", "cf"):
+        field_cmp(wg[k], wo[k], f"wall.{k}", 1e-9, 1e-300)
 
 
 def test_run_is_deterministic(jst):
