@@ -126,6 +126,7 @@ def bench_c4(args, jst):
     from purinecfd_amd import workloads
     cases = workloads.c4_batch()
     total = sum(w["ni"] * w["nj"] for w in cases)
+    per_case = []
     if args.throughput_only:
         plain_rate, dcl, dcd, l2 = float("nan"), [float("nan")], [float("nan")], {}
     else:
@@ -134,8 +135,10 @@ def bench_c4(args, jst):
         mixed = mixed_accuracy_pass(jst, cases, args.acc_steps)
         dcl = [abs(m[0]["cl"] - p[0]["cl"]) for m, p in zip(mixed, plain)]
         dcd = [abs(m[0]["cd"] - p[0]["cd"]) for m, p in zip(mixed, plain)]
-        l2 = {n: max(float(np.linalg.norm(m[1][n] - p[1][n]) / np.linalg.norm(p[1][n]))
-                     for m, p in zip(mixed, plain)) for n in ("rho", "u", "v", "p")}
+        rel = [{n: float(np.linalg.norm(m[1][n] - p[1][n]) / np.linalg.norm(p[1][n]))
+                for n in ("rho", "u", "v", "p")} for m, p in zip(mixed, plain)]
+        l2 = {n: max(r[n] for r in rel) for n in ("rho", "u", "v", "p")}
+        per_case = [[w["name"], a, b, max(r.values())] for w, a, b, r in zip(cases, dcl, dcd, rel)]
     # throughput: every case resident, autotuned tiles, HIP-graph replayed steps
     b = jst.Batch(hbm_bytes=8 << 30, max_cases=len(cases))
     for w in cases:
@@ -171,7 +174,9 @@ def bench_c4(args, jst):
         "speedup_vs_fp64_plain": value / plain_rate,
         "accuracy": {"steps": args.acc_steps, "max_abs_dcl": max(dcl), "max_abs_dcd": max(dcd),
                      "max_field_l2_rel": l2, "cases": len(cases),
-                     "reference": "FP64 plain GPU path (oracle-pinned) at the same step"},
+                     "reference": "FP64 plain GPU path (oracle-pinned) at the same step",
+                     "per_case_columns": ["case", "abs_dcl", "abs_dcd", "max_field_l2_rel"],
+                     "per_case": per_case},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": profiled_traffic("mix_march_inner", "c4_batch64"),

@@ -53,7 +53,7 @@ struct CaseDev {
     const double *vol, *inv_vol, *inv_wd2, *cx, *cy;
     double *ratio;                // frozen V/dt_local per cell (jst-rs dissipation.rs:44-52)
     double *localdt;              // frozen dt_local per cell (timestep.rs:24-47)
-    double *partial;              // per-tile partial sums of the step-end reduction
+    double *partial;              // step end: per-tile residual sums [ntiles] | tile min dt [ntiles]
     double *res_hist;             // residual ring buffer, res_cap entries
     int res_cap;
     CaseCtl *ctl;

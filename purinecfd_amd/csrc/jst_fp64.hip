@@ -9,7 +9,8 @@
 //                   central flux (convection.rs), Green-Gauss gradient (gradient.rs),
 //                   viscous/SA diffusion (viscous.rs), JST dissipation (dissipation.rs),
 //                   SA source (source.rs), update + non-physical check (solver.rs:123-185).
-//   post_kernel     end of step: L2 density residual (solver.rs:190-200) with a fixed-order
+//   post_tile_kernel / post_end_kernel
+//                   end of step: L2 density residual (solver.rs:190-200) with a fixed-order
 //                   reduction, next step's local/global dt (timestep.rs:24-57), step control.
 //   forces_kernel   wall force / CL / CD (no reference counterpart).
 #include <hip/hip_runtime.h>
@@ -352,8 +353,8 @@ struct StageSmem {
     double src[TI][TJ];
 };
 
-template <int TI, int TJ, int NT, bool TERMS>
-__global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ cases,
+template <int TI, int TJ, int NT, bool TERMS, int WPE>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void stage_kernel(const CaseDev *__restrict__ cases,
                                                    const TileDesc *__restrict__ tiles, int stage,
                                                    double alpha)
 {
@@ -649,37 +650,36 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
 }
 
 // ---------------------------------------------------------------------------------------
-// End of step (mode 1) or dt-only refresh (mode 0).
-//   mode 1: residual sqrt(sum (rho^{n+1}-rho^n)^2 / NI / NJ) with a deterministic
-//           reduction (per-tile tree, then fixed-order over tiles in the last block),
-//           totaltime += dt, step += 1, rotation advance, convergence / stop control.
-//   both:   localdt and V/dt of the new state (timestep.rs:24-47) and their global minimum
-//           by atomicMin on the bit pattern (order-independent, so deterministic).
+// End of step (mode 1) or dt-only refresh (mode 0), in two launches.
+//   post_tile_kernel, one block per tile:
+//     both:   localdt and V/dt of the new state (timestep.rs:24-47) per cell; the tile's
+//             minimum dt into partial[ntiles + t];
+//     mode 1: the tile's sum of (rho^{n+1}-rho^n)^2 (solver.rs:190-200) into partial[t].
+//   post_end_kernel, one block per case:
+//     mode 1: residual sqrt(sum / NI / NJ), the tile sums added in a fixed order (so the
+//             value does not depend on scheduling), totaltime += dt, step += 1, rotation
+//             advance, convergence / stop control;
+//     both:   dt of the next step = minimum over the tiles (exact, order-free).
+// Splitting the end off keeps device-scope fences and atomics out of the per-tile blocks.
 // ---------------------------------------------------------------------------------------
 template <int TI, int TJ, int NT>
-__global__ __launch_bounds__(NT) void post_kernel(const CaseDev *__restrict__ cases,
-                                                  const TileDesc *__restrict__ tiles, int mode)
+__global__ __launch_bounds__(NT) void post_tile_kernel(const CaseDev *__restrict__ cases,
+                                                       const TileDesc *__restrict__ tiles,
+                                                       int mode)
 {
     __shared__ double red_s[NT];
     __shared__ double red_m[NT];
-    __shared__ int last;
     const TileDesc tl = tiles[blockIdx.x];
     const CaseDev &C = cases[tl.c];
     const PhysConst &k = C.k;
-    CaseCtl *ctl = C.ctl;
+    const CaseCtl *ctl = C.ctl;
     const int ni = C.ni, nj = C.nj, pitch = C.pitch;
     const size_t plane = (size_t)ni * pitch;
     const int tid = threadIdx.x;
-    const int status = *(volatile int *)&ctl->status;
-    const unsigned long long ek = *(volatile unsigned long long *)&ctl->err_key;
-    const int rot = *(volatile int *)&ctl->rot;
-    if (mode == 1) {
-        if (status != JST_RUN_ACTIVE) return;
-        if (ek != JST_ERR_NONE) {
-            if (tid == 0) ctl->status = JST_RUN_NONPHYSICAL;
-            return;
-        }
-    }
+    const int status = *(volatile const int *)&ctl->status;
+    const unsigned long long ek = *(volatile const unsigned long long *)&ctl->err_key;
+    const int rot = *(volatile const int *)&ctl->rot;
+    if (mode == 1 && (status != JST_RUN_ACTIVE || ek != JST_ERR_NONE)) return;
     const int b1 = rot == 2 ? 0 : rot + 1;
     const double *unew = mode == 1 ? C.ubuf[b1] : C.ubuf[rot];
     const double *uold = C.ubuf[rot];
@@ -723,33 +723,50 @@ __global__ __launch_bounds__(NT) void post_kernel(const CaseDev *__restrict__ ca
     }
     if (tid == 0) {
         C.partial[tl.t] = red_s[0];
-        const double m = red_m[0];
-        if (m == m) atomicMin(&ctl->dt_acc, (unsigned long long)__double_as_longlong(m));
-        __threadfence();
-        const unsigned int prev = atomicAdd(&ctl->blocks_done, 1u);
-        last = prev == (unsigned int)(C.ntiles - 1);
+        C.partial[C.ntiles + tl.t] = red_m[0];
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void post_end_kernel(const CaseDev *__restrict__ cases, int mode)
+{
+    __shared__ double red_s[NT];
+    __shared__ double red_m[NT];
+    const CaseDev &C = cases[blockIdx.x];
+    const PhysConst &k = C.k;
+    volatile CaseCtl *v = C.ctl;
+    const int tid = threadIdx.x;
+    if (mode == 1) {
+        if (v->status != JST_RUN_ACTIVE) return;
+        if (v->err_key != JST_ERR_NONE) {
+            if (tid == 0) v->status = JST_RUN_NONPHYSICAL;
+            return;
+        }
+    }
     // fixed-order reduction of the per-tile partials
-    double s = 0.0;
-    for (int t = tid; t < C.ntiles; t += NT) s += ((volatile double *)C.partial)[t];
+    double s = 0.0, m = __builtin_huge_val();
+    for (int t = tid; t < C.ntiles; t += NT) {
+        s += C.partial[t];
+        m = fmin(m, C.partial[C.ntiles + t]);
+    }
     red_s[tid] = s;
+    red_m[tid] = m;
     __syncthreads();
     for (int h = NT / 2; h > 0; h >>= 1) {
-        if (tid < h) red_s[tid] = red_s[tid] + red_s[tid + h];
+        if (tid < h) {
+            red_s[tid] = red_s[tid] + red_s[tid + h];
+            red_m[tid] = fmin(red_m[tid], red_m[tid + h]);
+        }
         __syncthreads();
     }
     if (tid == 0) {
-        volatile CaseCtl *v = ctl;
-        const double new_dt = __longlong_as_double((long long)v->dt_acc);
         if (mode == 1) {
-            const double res = sqrt(red_s[0] / (double)ni / (double)nj);
+            const int rot = v->rot;
+            const double res = sqrt(red_s[0] / (double)C.ni / (double)C.nj);
             v->totaltime = v->totaltime + v->dt;
             const long long st = v->step + 1;
             v->step = st;
-            v->rot = b1;
+            v->rot = rot == 2 ? 0 : rot + 1;
             v->residual = res;
             C.res_hist[(st - 1) % C.res_cap] = res;
             if (st == 1) v->res_first = res;
@@ -761,10 +778,7 @@ __global__ __launch_bounds__(NT) void post_kernel(const CaseDev *__restrict__ ca
                 else if (st >= v->stop_step) v->status = JST_RUN_STOPPED;
             }
         }
-        v->dt = new_dt;
-        v->dt_acc = JST_INF_BITS;
-        v->blocks_done = 0u;
-        __threadfence();
+        v->dt = red_m[0];
     }
 }
 
@@ -865,14 +879,22 @@ hipError_t launch_geometry(const double *xy, int n_theta, int ni, int nj, int pi
     return hipGetLastError();
 }
 
+// Two builds of the stage: one wave per SIMD and no register spills (shortest time per
+// tile: a case with at most one tile per CU is bound by that), and two waves per SIMD with a
+// few spilled registers (more latency hidden once tiles queue behind each other: 1.2x on a
+// 524 288-cell case).  Same arithmetic, so the results are bitwise the same.
 hipError_t launch_stage_fp64(const CaseDev *cases, const TileDesc *tiles, int ntiles, int stage,
                              double alpha, bool terms, hipStream_t s)
 {
+    const bool two_waves = ntiles > FP64_TWO_WAVE_TILES;
     if (terms)
-        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, true>), dim3(ntiles),
+        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, true, 1>), dim3(ntiles),
+                           dim3(FP64_NT), 0, s, cases, tiles, stage, alpha);
+    else if (two_waves)
+        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, false, 2>), dim3(ntiles),
                            dim3(FP64_NT), 0, s, cases, tiles, stage, alpha);
     else
-        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, false>), dim3(ntiles),
+        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, false, 1>), dim3(ntiles),
                            dim3(FP64_NT), 0, s, cases, tiles, stage, alpha);
     return hipGetLastError();
 }
@@ -880,8 +902,9 @@ hipError_t launch_stage_fp64(const CaseDev *cases, const TileDesc *tiles, int nt
 hipError_t launch_post_fp64(const CaseDev *cases, const TileDesc *tiles, int ntiles, int mode,
                             hipStream_t s)
 {
-    hipLaunchKernelGGL((post_kernel<FP64_TI, FP64_TJ, FP64_NT>), dim3(ntiles), dim3(FP64_NT), 0, s,
-                       cases, tiles, mode);
+    hipLaunchKernelGGL((post_tile_kernel<FP64_TI, FP64_TJ, FP64_NT>), dim3(ntiles), dim3(FP64_NT),
+                       0, s, cases, tiles, mode);
+    hipLaunchKernelGGL((post_end_kernel<256>), dim3(1), dim3(256), 0, s, cases, mode);
     return hipGetLastError();
 }
 

@@ -698,6 +698,9 @@ template <class T> jst_status ctl_set(jst_ctx *x, T CaseCtl::*field, T value)
 
 static const double RK_ALPHA[5] = {0.25, 1.0 / 6.0, 0.375, 0.5, 1.0}; // config.rs:295
 
+// Steps are launched directly (five stages, then the two step-end launches).  Replaying them
+// from a HIP graph was measured as no faster (a small case's step is bound by the per-kernel
+// turnaround of about 10 us on the device either way, and slightly slower in graph form).
 jst_status launch_steps(jst_ctx *x, int n)
 {
     for (int s = 0; s < n; ++s) {

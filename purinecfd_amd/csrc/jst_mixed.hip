@@ -1,11 +1,13 @@
 // jst_mixed.hip — FP32-mixed path for gfx950 (batched: one launch covers many cases).
 //
-// The stage stencils read an FP32 shadow of dU = U - U_inf and compute in FP32; the RK
-// update U^(k) = U^n - alpha dt R / V is accumulated in FP64 from the FP64 U^n.  Face fluxes
-// are evaluated in perturbation form (F(U) - F(U_inf)) . n: the freestream part cancels
-// exactly over a closed cell (metric closure, jst-rs geometry.rs:17-18) and so never enters
-// a single-precision sum.  Every face flux is computed once per tile; results depend only
-// on the case's own data, not on the tile shape or on the other cases of the launch.
+// The stage stencils read FP32 perturbations dU = U - U_inf and compute in FP32.  RK stages
+// 1-4 write the FP32 stage state dU^(k) = dU^n - a_k dt R / V from the FP32 base dU^n; stage 5
+// accumulates U^{n+1} = U^n - dt R / V in FP64 from the FP64 U^n and refreshes the FP32
+// base.  Face fluxes are evaluated in perturbation form (F(U) - F(U_inf)) . n: the
+// freestream part cancels exactly over a closed cell (metric closure, jst-rs
+// geometry.rs:17-18) and so never enters a single-precision sum.  Every face flux is
+// computed once per work item; results depend only on the case's own data, not on the
+// launch grid or on the other cases of the launch.
 //
 // Scheme (same as the FP64 plain path): jst-rs boundary.rs (ghosts), convection.rs,
 // gradient.rs, viscous.rs, dissipation.rs, source.rs, solver.rs:123-200 (update, residual).
@@ -233,9 +235,10 @@ __device__ __forceinline__ float sensor32(float pinf4, float dm, float d0, float
     return fabsf((dm - 2.0f * d0 + dp) * frcp(pinf4 + dm + 2.0f * d0 + dp));
 }
 
-// Lane l of a wave owns column J0 + l - 3 for 3 <= l < 61 (58 columns); lanes 0..2 and 61..63
-// carry the periodic halo (the widest stencil, a face's sensors, reaches 3 columns out).  Values of the j-neighbours move between lanes with DPP wave shifts,
-// which read every lane, so they are only ever called from wave-uniform control flow.
+// Lane l of a wave owns column J0 + l - 3 for 3 <= l < 61 (58 columns); lanes 0..2 and
+// 61..63 carry the periodic halo (the widest stencil, a face's sensors, reaches 3 columns
+// out).  Values of the j-neighbours move between lanes with DPP wave shifts, which read
+// every lane, so they are only ever called from wave-uniform control flow.
 __device__ __forceinline__ float from_left(float x) // lane l receives lane l-1
 {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));

@@ -147,6 +147,31 @@ def test_run_is_deterministic(jst):
     assert a.totaltime == b.totaltime
 
 
+def test_large_case_matches_oracle(jst, oracle_lib):
+    """A 513 x 257-point NACA case of the uneven batch (131 072 cells, 512 stage tiles, so the
+    two-waves-per-SIMD stage build and the multi-tile step end run): 40 steps, then one
+    residual evaluation, against the oracle at 1e-10."""
+    from purinecfd_amd import workloads
+    w = next(c for c in workloads.c4_batch() if (c["ni"], c["nj"]) == (256, 512))
+    s = jst.Solver(w["config"], w["mesh"])
+    o = oracle_lib.OracleSolver(json.loads(w["config"]), w["mesh"])
+    hist = []
+    rep = s.run(40, on_step=lambda k, r: hist.append(r))
+    orep = o.run(40)
+    assert rep.steps == 40 and orep["steps"] == 40
+    field_cmp(np.array(hist), orep["history"], "residual history", 1e-9)
+    assert abs(rep.totaltime - o.totaltime) <= 1e-12 * o.totaltime
+    s.residual_terms()
+    o.timestep()
+    o.residual_terms()
+    for term in ("Fc", "Fv", "Fd"):
+        for k in range(5):
+            field_cmp(s.get(f"{term}{k}"), o.get(f"{term}{k}"), f"{term}{k}", 1e-10, 1e-280)
+    field_cmp(s.get("S"), o.get("S"), "S", 1e-10, 1e-280)
+    for name in ("rho", "p", "u", "v", "nut"):
+        field_cmp(s.get(name), o.get(name), name, 1e-10, 1e-300)
+
+
 def test_meshes_of_several_shapes_stay_physical(jst, oracle_lib):
     """properties.rs:158-172, with the oracle as the reference on each shape."""
     for rings, nj, stretch in ((8, 16, 1.0), (13, 40, 1.0), (21, 64, 1.25)):
