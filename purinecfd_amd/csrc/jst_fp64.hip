@@ -353,8 +353,8 @@ struct StageSmem {
     double src[TI][TJ];
 };
 
-template <int TI, int TJ, int NT, bool TERMS, int WPE>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void stage_kernel(const CaseDev *__restrict__ cases,
+template <int TI, int TJ, int NT, bool TERMS>
+__global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ cases,
                                                    const TileDesc *__restrict__ tiles, int stage,
                                                    double alpha)
 {
@@ -509,6 +509,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     __syncthreads();
 
     // ---- phase 3: face fluxes, residual and update of the tile's own cells ----
+    // Per cell, the per-face scalars come first (face-average normal velocity and pressure
+    // of the convective flux, the JST coefficients); the five components then follow one at
+    // a time with the reference expressions, which keeps few values live at once.
     const double gamma = k.gamma, half_cfl = 0.5 * k.cfl;
     for (int o = tid; o < TI * TJ; o += NT) {
         const int ti = o / TJ, tj = o - ti * TJ;
@@ -517,121 +520,104 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE))) void 
         const int R = ti + 3, Q = tj + 3;    // R3
         const int R1 = ti + 1, Q1 = tj + 1;  // R1
         const int jp = j + 1 < nj ? j + 1 : 0;
-        double fc[5], fv[5], fd[5];
-        double a[5], b[5], up[5], dn[5], rt[5], lf[5];
-        // convection (convection.rs:46-80): tau faces i (dn) and i+1 (up), n faces j (lf), j+1 (rt)
+        // faces: 0 = tau face i (dn), 1 = tau face i+1 (up), 2 = n face j (lf), 3 = n face j+1 (rt)
         const size_t c_dn = cidx(i, j, pitch), c_up = cidx(i + 1, j, pitch), c_rt = cidx(i, jp, pitch);
-        const double dnx = C.tnx[c_dn], dny = C.tny[c_dn], upx = C.tnx[c_up], upy = C.tny[c_up];
-        const double lfx = C.nnx[c_dn], lfy = C.nny[c_dn], rtx = C.nnx[c_rt], rty = C.nny[c_rt];
-#pragma unroll
-        for (int m = 0; m < 5; ++m) { a[m] = sm.u[m][R - 1][Q]; b[m] = sm.u[m][R][Q]; }
-        euler_flux(a, b, dnx, dny, gamma, dn);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) a[m] = sm.u[m][R + 1][Q];
-        euler_flux(b, a, upx, upy, gamma, up);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) a[m] = sm.u[m][R][Q - 1];
-        euler_flux(a, b, lfx, lfy, gamma, lf);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) a[m] = sm.u[m][R][Q + 1];
-        euler_flux(b, a, rtx, rty, gamma, rt);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) fc[m] = up[m] - dn[m] + rt[m] - lf[m];
+        const double nx[4] = {C.tnx[c_dn], C.tnx[c_up], C.nnx[c_dn], C.nnx[c_rt]};
+        const double ny[4] = {C.tny[c_dn], C.tny[c_up], C.nny[c_dn], C.nny[c_rt]};
+        // the two cells of each face in R3: first (R - 1 | R | R | R), second (R | R + 1 | R | R)
+        const int ar[4] = {R - 1, R, R, R}, ac[4] = {Q, Q, Q - 1, Q};
+        const int br[4] = {R, R + 1, R, R}, bc[4] = {Q, Q, Q, Q + 1};
 
-        // viscous faces (viscous.rs:128-164): 0.5 (D_a + D_b) . n
+        // convection.rs:25-40 per face: fu = (U_a + U_b)/2, u, v, p, v_n
+        double vn[4], pf[4];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const double cx0 = sm.d[m][R1][Q1], cy0 = sm.d[4 + m][R1][Q1];
-            dn[m + 1] = (sm.d[m][R1 - 1][Q1] + cx0) * (0.5 * dnx) + (sm.d[4 + m][R1 - 1][Q1] + cy0) * (0.5 * dny);
-            up[m + 1] = (cx0 + sm.d[m][R1 + 1][Q1]) * (0.5 * upx) + (cy0 + sm.d[4 + m][R1 + 1][Q1]) * (0.5 * upy);
-            lf[m + 1] = (sm.d[m][R1][Q1 - 1] + cx0) * (0.5 * lfx) + (sm.d[4 + m][R1][Q1 - 1] + cy0) * (0.5 * lfy);
-            rt[m + 1] = (cx0 + sm.d[m][R1][Q1 + 1]) * (0.5 * rtx) + (cy0 + sm.d[4 + m][R1][Q1 + 1]) * (0.5 * rty);
+        for (int f = 0; f < 4; ++f) {
+            const double rho = 0.5 * (sm.u[0][ar[f]][ac[f]] + sm.u[0][br[f]][bc[f]]);
+            const double inv_rho = 1.0 / rho;
+            const double u = 0.5 * (sm.u[1][ar[f]][ac[f]] + sm.u[1][br[f]][bc[f]]) * inv_rho;
+            const double v = 0.5 * (sm.u[2][ar[f]][ac[f]] + sm.u[2][br[f]][bc[f]]) * inv_rho;
+            const double rho_e = 0.5 * (sm.u[3][ar[f]][ac[f]] + sm.u[3][br[f]][bc[f]]);
+            pf[f] = (gamma - 1.0) * (rho_e - rho * (u * u + v * v) * 0.5);
+            vn[f] = nx[f] * u + ny[f] * v;
         }
-        // component 0 of D is identically zero: (0 + 0) * (0.5 n) terms sum to +0
-        dn[0] = 0.0 * (0.5 * dnx) + 0.0 * (0.5 * dny);
-        up[0] = 0.0 * (0.5 * upx) + 0.0 * (0.5 * upy);
-        lf[0] = 0.0 * (0.5 * lfx) + 0.0 * (0.5 * lfy);
-        rt[0] = 0.0 * (0.5 * rtx) + 0.0 * (0.5 * rty);
-#pragma unroll
-        for (int m = 0; m < 5; ++m) fv[m] = up[m] - dn[m] + rt[m] - lf[m];
 
-        // JST dissipation (dissipation.rs:44-185)
+        // dissipation.rs:44-185 per face: eps2 from the 4-cell sensor maximum, eps4, lambda
+        double e2[4], e4[4], lam[4];
         {
             const int ri = ti + 2; // sensor_i row of cell i (rows start at i0-2)
-            // tau face i: sensors i-2..i+1, U rows i-2..i+1
-            double m4 = fmax(fmax(fmax(sm.si[ri - 2][tj], sm.si[ri - 1][tj]), sm.si[ri][tj]), sm.si[ri + 1][tj]);
-            double e2 = k.k2 * m4, e4 = fmax(k.k4 - e2, 0.0);
-            double lam = half_cfl * (sm.ratio[R1][Q1] + sm.ratio[R1 - 1][Q1]);
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                const double um2 = sm.u[m][R - 2][Q], um1 = sm.u[m][R - 1][Q], u0 = sm.u[m][R][Q], up1 = sm.u[m][R + 1][Q];
-                const double d1 = u0 - um1;
-                const double d3 = up1 - u0 * 3.0 + um1 * 3.0 - um2;
-                dn[m] = (d1 * e2 - d3 * e4) * lam;
-            }
-            // tau face i+1: sensors i-1..i+2, U rows i-1..i+2
-            m4 = fmax(fmax(fmax(sm.si[ri - 1][tj], sm.si[ri][tj]), sm.si[ri + 1][tj]), sm.si[ri + 2][tj]);
-            e2 = k.k2 * m4;
-            e4 = fmax(k.k4 - e2, 0.0);
-            lam = half_cfl * (sm.ratio[R1 + 1][Q1] + sm.ratio[R1][Q1]);
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                const double um2 = sm.u[m][R - 1][Q], um1 = sm.u[m][R][Q], u0 = sm.u[m][R + 1][Q], up1 = sm.u[m][R + 2][Q];
-                const double d1 = u0 - um1;
-                const double d3 = up1 - u0 * 3.0 + um1 * 3.0 - um2;
-                up[m] = (d1 * e2 - d3 * e4) * lam;
-            }
             const int cj = tj + 2; // sensor_j column of cell j (columns start at j0-2)
-            // n face j: sensors j-2..j+1, U cols j-2..j+1
-            m4 = fmax(fmax(fmax(sm.sj[ti][cj - 2], sm.sj[ti][cj - 1]), sm.sj[ti][cj]), sm.sj[ti][cj + 1]);
-            e2 = k.k2 * m4;
-            e4 = fmax(k.k4 - e2, 0.0);
-            lam = half_cfl * (sm.ratio[R1][Q1] + sm.ratio[R1][Q1 - 1]);
+            double m4[4];
+            m4[0] = fmax(fmax(fmax(sm.si[ri - 2][tj], sm.si[ri - 1][tj]), sm.si[ri][tj]), sm.si[ri + 1][tj]);
+            m4[1] = fmax(fmax(fmax(sm.si[ri - 1][tj], sm.si[ri][tj]), sm.si[ri + 1][tj]), sm.si[ri + 2][tj]);
+            m4[2] = fmax(fmax(fmax(sm.sj[ti][cj - 2], sm.sj[ti][cj - 1]), sm.sj[ti][cj]), sm.sj[ti][cj + 1]);
+            m4[3] = fmax(fmax(fmax(sm.sj[ti][cj - 1], sm.sj[ti][cj]), sm.sj[ti][cj + 1]), sm.sj[ti][cj + 2]);
+            lam[0] = half_cfl * (sm.ratio[R1][Q1] + sm.ratio[R1 - 1][Q1]);
+            lam[1] = half_cfl * (sm.ratio[R1 + 1][Q1] + sm.ratio[R1][Q1]);
+            lam[2] = half_cfl * (sm.ratio[R1][Q1] + sm.ratio[R1][Q1 - 1]);
+            lam[3] = half_cfl * (sm.ratio[R1][Q1 + 1] + sm.ratio[R1][Q1]);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                const double um2 = sm.u[m][R][Q - 2], um1 = sm.u[m][R][Q - 1], u0 = sm.u[m][R][Q], up1 = sm.u[m][R][Q + 1];
-                const double d1 = u0 - um1;
-                const double d3 = up1 - u0 * 3.0 + um1 * 3.0 - um2;
-                lf[m] = (d1 * e2 - d3 * e4) * lam;
+            for (int f = 0; f < 4; ++f) {
+                e2[f] = k.k2 * m4[f];
+                e4[f] = fmax(k.k4 - e2[f], 0.0);
             }
-            // n face j+1: sensors j-1..j+2, U cols j-1..j+2
-            m4 = fmax(fmax(fmax(sm.sj[ti][cj - 1], sm.sj[ti][cj]), sm.sj[ti][cj + 1]), sm.sj[ti][cj + 2]);
-            e2 = k.k2 * m4;
-            e4 = fmax(k.k4 - e2, 0.0);
-            lam = half_cfl * (sm.ratio[R1][Q1 + 1] + sm.ratio[R1][Q1]);
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                const double um2 = sm.u[m][R][Q - 1], um1 = sm.u[m][R][Q], u0 = sm.u[m][R][Q + 1], up1 = sm.u[m][R][Q + 2];
-                const double d1 = u0 - um1;
-                const double d3 = up1 - u0 * 3.0 + um1 * 3.0 - um2;
-                rt[m] = (d1 * e2 - d3 * e4) * lam;
-            }
-#pragma unroll
-            for (int m = 0; m < 5; ++m) fd[m] = up[m] + rt[m] - dn[m] - lf[m];
         }
+        // 4-cell stencils um2 | um1 || u0 | up1 across each face (R3 coordinates)
+        const int sr[4][4] = {{R - 2, R - 1, R, R + 1}, {R - 1, R, R + 1, R + 2}, {R, R, R, R}, {R, R, R, R}};
+        const int sc[4][4] = {{Q, Q, Q, Q}, {Q, Q, Q, Q}, {Q - 2, Q - 1, Q, Q + 1}, {Q - 1, Q, Q + 1, Q + 2}};
+        // viscous faces in R1: the neighbour across each face
+        const int vr[4] = {R1 - 1, R1 + 1, R1, R1}, vc[4] = {Q1, Q1, Q1 - 1, Q1 + 1};
 
         const size_t cc = cidx(i, j, pitch);
-        if (TERMS) {
+        const double a_dt = k.lts ? alpha * C.localdt[cc] : alpha * dt;
+        const double scl = a_dt * C.inv_vol[cc];
+        double un[4];
 #pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                C.dbg[(DBG_FC + m) * plane + cc] = fc[m];
-                C.dbg[(DBG_FV + m) * plane + cc] = fv[m];
-                C.dbg[(DBG_FD + m) * plane + cc] = fd[m];
+        for (int m = 0; m < 5; ++m) {
+            double fl[4], vf[4], df[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f) {
+                // convection.rs:34-39, component m of F(fu) . n
+                const double fu = 0.5 * (sm.u[m][ar[f]][ac[f]] + sm.u[m][br[f]][bc[f]]);
+                if (m == 0) fl[f] = fu * vn[f];
+                else if (m == 1) fl[f] = fu * vn[f] + pf[f] * nx[f];
+                else if (m == 2) fl[f] = fu * vn[f] + pf[f] * ny[f];
+                else if (m == 3) fl[f] = (fu + pf[f]) * vn[f];
+                else fl[f] = fu * vn[f];
+                // viscous.rs:128-164: 0.5 (D_a + D_b) . n; component 0 of D is identically
+                // zero, so its (0 + 0) * (0.5 n) terms sum to +0
+                if (m == 0) {
+                    vf[f] = 0.0 * (0.5 * nx[f]) + 0.0 * (0.5 * ny[f]);
+                } else {
+                    const double cx0 = sm.d[m - 1][R1][Q1], cy0 = sm.d[3 + m][R1][Q1];
+                    const double cx1 = sm.d[m - 1][vr[f]][vc[f]], cy1 = sm.d[3 + m][vr[f]][vc[f]];
+                    // the lower/left neighbour is the first operand of each sum
+                    vf[f] = (f == 0 || f == 2) ? (cx1 + cx0) * (0.5 * nx[f]) + (cy1 + cy0) * (0.5 * ny[f])
+                                               : (cx0 + cx1) * (0.5 * nx[f]) + (cy0 + cy1) * (0.5 * ny[f]);
+                }
+                // dissipation.rs:140-170: D = (eps2 d1 - eps4 d3) lambda
+                const double um2 = sm.u[m][sr[f][0]][sc[f][0]], um1 = sm.u[m][sr[f][1]][sc[f][1]];
+                const double u0 = sm.u[m][sr[f][2]][sc[f][2]], up1 = sm.u[m][sr[f][3]][sc[f][3]];
+                const double d1 = u0 - um1;
+                const double d3 = up1 - u0 * 3.0 + um1 * 3.0 - um2;
+                df[f] = (d1 * e2[f] - d3 * e4[f]) * lam[f];
             }
-        } else {
-            // solver.rs:123-145: U = U_former - (Fc - Fv - Fd - S) * (alpha dt / V)
-            double r[5];
-#pragma unroll
-            for (int m = 0; m < 5; ++m) r[m] = fc[m] - fv[m] - fd[m];
-            r[4] -= sm.src[ti][tj];
-            const double a_dt = k.lts ? alpha * C.localdt[cc] : alpha * dt;
-            const double sc = a_dt * C.inv_vol[cc];
-            double un[5];
-#pragma unroll
-            for (int m = 0; m < 5; ++m) {
-                un[m] = former[m * plane + cc] - r[m] * sc;
-                dst[m * plane + cc] = un[m];
+            const double fc = fl[1] - fl[0] + fl[3] - fl[2];   // convection.rs:77
+            const double fv = vf[1] - vf[0] + vf[3] - vf[2];   // viscous.rs:160
+            const double fd = df[1] + df[3] - df[0] - df[2];   // dissipation.rs:182
+            if (TERMS) {
+                C.dbg[(DBG_FC + m) * plane + cc] = fc;
+                C.dbg[(DBG_FV + m) * plane + cc] = fv;
+                C.dbg[(DBG_FD + m) * plane + cc] = fd;
+            } else {
+                // solver.rs:123-145: U = U_former - (Fc - Fv - Fd - S) * (alpha dt / V)
+                double r = fc - fv - fd;
+                if (m == 4) r -= sm.src[ti][tj];
+                const double v = former[m * plane + cc] - r * scl;
+                dst[m * plane + cc] = v;
+                if (m < 4) un[m] = v;
             }
+        }
+        if (!TERMS) {
             // solver.rs:151-171: first row-major cell with rho or p NaN or <= 1e-15
             bool bad = isnan(un[0]) || un[0] <= 1e-15;
             if (!bad) {
@@ -879,22 +865,14 @@ hipError_t launch_geometry(const double *xy, int n_theta, int ni, int nj, int pi
     return hipGetLastError();
 }
 
-// Two builds of the stage: one wave per SIMD and no register spills (shortest time per
-// tile: a case with at most one tile per CU is bound by that), and two waves per SIMD with a
-// few spilled registers (more latency hidden once tiles queue behind each other: 1.2x on a
-// 524 288-cell case).  Same arithmetic, so the results are bitwise the same.
 hipError_t launch_stage_fp64(const CaseDev *cases, const TileDesc *tiles, int ntiles, int stage,
                              double alpha, bool terms, hipStream_t s)
 {
-    const bool two_waves = ntiles > FP64_TWO_WAVE_TILES;
     if (terms)
-        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, true, 1>), dim3(ntiles),
-                           dim3(FP64_NT), 0, s, cases, tiles, stage, alpha);
-    else if (two_waves)
-        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, false, 2>), dim3(ntiles),
+        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, true>), dim3(ntiles),
                            dim3(FP64_NT), 0, s, cases, tiles, stage, alpha);
     else
-        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, false, 1>), dim3(ntiles),
+        hipLaunchKernelGGL((stage_kernel<FP64_TI, FP64_TJ, FP64_NT, false>), dim3(ntiles),
                            dim3(FP64_NT), 0, s, cases, tiles, stage, alpha);
     return hipGetLastError();
 }

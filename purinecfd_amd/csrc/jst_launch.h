@@ -9,7 +9,6 @@
 #define FP64_TI 8
 #define FP64_TJ 32
 #define FP64_NT 256
-#define FP64_TWO_WAVE_TILES 256 // more tiles than this: the two-waves-per-SIMD stage build
 
 namespace jst {
 

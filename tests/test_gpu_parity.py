@@ -148,9 +148,9 @@ def test_run_is_deterministic(jst):
 
 
 def test_large_case_matches_oracle(jst, oracle_lib):
-    """A 513 x 257-point NACA case of the uneven batch (131 072 cells, 512 stage tiles, so the
-    two-waves-per-SIMD stage build and the multi-tile step end run): 40 steps, then one
-    residual evaluation, against the oracle at 1e-10."""
+    """A 513 x 257-point NACA case of the uneven batch (131 072 cells, 512 stage tiles, more
+    than one per CU, and a 512-tile step end): 40 steps, then one residual evaluation,
+    against the oracle at 1e-10."""
     from purinecfd_amd import workloads
     w = next(c for c in workloads.c4_batch() if (c["ni"], c["nj"]) == (256, 512))
     s = jst.Solver(w["config"], w["mesh"])
