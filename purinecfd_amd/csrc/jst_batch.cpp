@@ -1009,22 +1009,30 @@ static jst_status tune_impl(jst_batch *b, int trial_steps, bool variants, int *v
 {
     if (!b || trial_steps <= 0) return fail(JST_ERR_ARG, "bad arguments");
     int best_v = b->variant, best_g = b->grid;
-    double best = 1e300;
     const int v0 = variants ? 0 : b->variant, v1 = variants ? MIX_NVARIANTS : b->variant + 1;
-    for (int v = v0; v < v1; ++v) {
-        // grid 0: one workgroup per tile; otherwise a persistent grid of k workgroups per CU
-        for (int g : {0, 256 * 2, 256 * 4}) {
-            jst_status st = jst_batch_set_tiling(b, v, g);
-            if (st != JST_OK) return st;
-            double ms = 0.0;
-            if ((st = jst_batch_time_steps(b, trial_steps, &ms, nullptr)) != JST_OK) return st;
-            if (ms < best) {
-                best = ms;
-                best_v = v;
-                best_g = g;
+    // grid 0: one workgroup per four work items; otherwise a persistent grid of k workgroups
+    // per CU.  Each candidate is timed in two rounds and keeps its faster time, so that a
+    // one-off stall in one timing does not decide the choice.
+    const int grids[3] = {0, 256 * 2, 256 * 4};
+    std::vector<double> t((size_t)(v1 - v0) * 3, 1e300);
+    for (int round = 0; round < 2; ++round)
+        for (int v = v0; v < v1; ++v)
+            for (int q = 0; q < 3; ++q) {
+                jst_status st = jst_batch_set_tiling(b, v, grids[q]);
+                if (st != JST_OK) return st;
+                double ms = 0.0;
+                if ((st = jst_batch_time_steps(b, trial_steps, &ms, nullptr)) != JST_OK) return st;
+                double &m = t[(size_t)(v - v0) * 3 + q];
+                m = std::min(m, ms);
             }
-        }
-    }
+    double best = 1e300;
+    for (int v = v0; v < v1; ++v)
+        for (int q = 0; q < 3; ++q)
+            if (t[(size_t)(v - v0) * 3 + q] < best) {
+                best = t[(size_t)(v - v0) * 3 + q];
+                best_v = v;
+                best_g = grids[q];
+            }
     jst_status st = jst_batch_set_tiling(b, best_v, best_g);
     if (st != JST_OK) return st;
     if (variant_out) *variant_out = best_v;
