@@ -213,8 +213,9 @@ jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, doub
  * work items).  The grid never changes results; the segment length changes them only at
  * FP32 rounding level (where a segment's edge rows are evaluated). */
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);
-/* Autotuner: times every variant and grid size on the current batch for trial_steps steps
- * each (the steps are real work), keeps the fastest; returns the choice. */
+/* Autotuner: times every variant with three grid sizes (0, 512, 1024) on the current batch
+ * for trial_steps steps each, in two rounds (each candidate keeps its faster time; the steps
+ * are real work and advance the cases), and keeps the fastest; returns the choice. */
 jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant, int *grid);
 /* Benchmark hook: nsteps steps of every running case with convergence stops disabled;
  * total_ms from HIP events around the region, stage_ms (may be NULL) the mean duration of

@@ -1,8 +1,9 @@
 """GPU FP64 plain path against the reference golden vectors and the CPU oracle.
 
 Mirrors jst-rs tests/golden.rs on the GPU through the C ABI, then checks the GPU against the
-oracle on the BASELINE cylinder case (per-residual-evaluation error <= 1e-10, converged
-CL/CD to 1e-8) and on the reference's robustness properties (tests/properties.rs).
+oracle on a 40 x 17 cylinder case (per-residual-evaluation error <= 1e-10, converged CL/CD
+to 1e-8), on a 513 x 257 NACA case and on the reference's robustness properties
+(tests/properties.rs).
 """
 from __future__ import annotations
 
@@ -104,7 +105,7 @@ def test_final_state(golden, jst):
 
 
 def test_cylinder_c1_matches_oracle(jst, oracle_lib):
-    """BASELINE cylinder case (40 x 17 points, M 0.2, Re 1e3, CFL 2, k4 1/64, nu~/nu 3):
+    """Cylinder case (40 x 17 points, M 0.2, Re 1e3, CFL 2, k4 1/64, nu~/nu 3):
     per-step residual history, every per-residual evaluation along the way and the
     converged CL/CD after 2000 iterations against the oracle."""
     from purinecfd_amd import workloads
