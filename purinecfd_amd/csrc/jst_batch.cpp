@@ -105,8 +105,9 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
-    L.rpart = carve((size_t)((ni + 7) / 8) * nj * 8);
-    L.dpart = carve((size_t)((ni + 7) / 8) * nj * 8);
+    const size_t nparts = (size_t)((ni + 7) / 8) * ((nj + MIX_COLS - 1) / MIX_COLS);
+    L.rpart = carve(nparts * 8);
+    L.dpart = carve(nparts * 8);
     L.res_hist = carve((size_t)res_cap * 8);
     L.posts = carve((size_t)npost * sizeof(PostDesc));
     L.scratch = carve((4 * (size_t)nj + 2) * 8); // wall-force terms [4][nj] + 2

@@ -341,15 +341,22 @@ struct StageSmem {
     static constexpr int H3 = TI + 6, W3 = TJ + 6; // R3 rectangle
     static constexpr int H1 = TI + 2, W1 = TJ + 2; // R1 rectangle
     double u[5][H3][W3];
-    double p[H3][W3];
-    double t[H3][W3];
-    double vx[H3][W3];
-    double vy[H3][W3];
-    double nu[H3][W3];
+    // The primitives on R3 are read in phase 2 only, and the viscous tensor is written at the
+    // end of phase 2 (after a barrier) and read in phase 3: they share storage, which brings
+    // an 8 x 32 tile to 53 184 B, three workgroups per CU.
+    union {
+        struct {
+            double p[H3][W3];
+            double t[H3][W3];
+            double vx[H3][W3];
+            double vy[H3][W3];
+            double nu[H3][W3];
+        } q;
+        double d[8][H1][W1]; // viscous tensor on R1
+    };
     double si[TI + 4][TJ]; // sensor_i rows i0-2 .. i0+TI+1
     double sj[TI][TJ + 4]; // sensor_j cols j0-2 .. j0+TJ+1
     double ratio[H1][W1];  // V/dt rows clamp(i0-1 .. i0+TI), cols j0-1 .. j0+TJ (wrapped)
-    double d[8][H1][W1];   // viscous tensor on R1
     double src[TI][TJ];
 };
 
@@ -428,11 +435,11 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
         }
 #pragma unroll
         for (int m = 0; m < 5; ++m) sm.u[m][r][c] = u[m];
-        sm.p[r][c] = q.p;
-        sm.t[r][c] = q.t;
-        sm.vx[r][c] = q.vx;
-        sm.vy[r][c] = q.vy;
-        sm.nu[r][c] = q.nut;
+        sm.q.p[r][c] = q.p;
+        sm.q.t[r][c] = q.t;
+        sm.q.vx[r][c] = q.vx;
+        sm.q.vy[r][c] = q.vy;
+        sm.q.nu[r][c] = q.nut;
     }
     // frozen V/dt on R1: rows clamped to [0, NI-1] (dissipation.rs:59), columns wrapped.
     for (int idx = tid; idx < H1 * W1; idx += NT) {
@@ -446,13 +453,22 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
     // ---- phase 2: sensors, gradients + viscous tensor on R1, SA source on the tile ----
     for (int idx = tid; idx < (TI + 4) * TJ; idx += NT) {
         const int r = idx / TJ, c = idx - r * TJ; // cell (i0-2+r, j0+c) -> R3 (r+1, c+3)
-        sm.si[r][c] = pressure_sensor(sm.p[r][c + 3], sm.p[r + 1][c + 3], sm.p[r + 2][c + 3]);
+        sm.si[r][c] = pressure_sensor(sm.q.p[r][c + 3], sm.q.p[r + 1][c + 3], sm.q.p[r + 2][c + 3]);
     }
     for (int idx = tid; idx < TI * (TJ + 4); idx += NT) {
         const int r = idx / (TJ + 4), c = idx - r * (TJ + 4); // cell (i0+r, j0-2+c)
-        sm.sj[r][c] = pressure_sensor(sm.p[r + 3][c], sm.p[r + 3][c + 1], sm.p[r + 3][c + 2]);
+        sm.sj[r][c] = pressure_sensor(sm.q.p[r + 3][c], sm.q.p[r + 3][c + 1], sm.q.p[r + 3][c + 2]);
     }
-    for (int idx = tid; idx < H1 * W1; idx += NT) {
+    // the viscous tensor of each R1 cell stays in registers until every thread has read the
+    // primitives it overwrites
+    constexpr int NR1 = (H1 * W1 + NT - 1) / NT;
+    double dreg[NR1][8];
+    bool dset[NR1];
+#pragma unroll
+    for (int slot = 0; slot < NR1; ++slot) {
+        const int idx = tid + slot * NT;
+        dset[slot] = false;
+        if (idx >= H1 * W1) continue;
         const int r = idx / W1, c = idx - r * W1;
         const int i = i0 - 1 + r, jj = j0 - 1 + c;
         const int R = r + 2, Q = c + 2; // R3 coordinates of the cell
@@ -472,7 +488,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
             const double lfx = C.nnx[cidx(ie, jw, pitch)], lfy = C.nny[cidx(ie, jw, pitch)];
             const double rtx = C.nnx[cidx(ie, jp, pitch)], rty = C.nny[cidx(ie, jp, pitch)];
             const double inv_v = C.inv_vol[cidx(ie, jw, pitch)];
-            const double(*fld[4])[W3] = {sm.vx, sm.vy, sm.t, sm.nu};
+            const double(*fld[4])[W3] = {sm.q.vx, sm.q.vy, sm.q.t, sm.q.nu};
 #pragma unroll
             for (int f = 0; f < 4; ++f) {
                 const double cc = fld[f][Re][Q];
@@ -488,10 +504,10 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
                 g[5] = 0.0;
             }
         }
-        double d[8], chi, fv1;
-        diffusion_tensor(k, sm.t[R][Q], sm.u[4][R][Q], sm.vx[R][Q], sm.vy[R][Q], g, d, chi, fv1);
-#pragma unroll
-        for (int m = 0; m < 8; ++m) sm.d[m][r][c] = d[m];
+        double chi, fv1;
+        diffusion_tensor(k, sm.q.t[R][Q], sm.u[4][R][Q], sm.q.vx[R][Q], sm.q.vy[R][Q], g,
+                         dreg[slot], chi, fv1);
+        dset[slot] = true;
         const bool own = r >= 1 && r <= TI && c >= 1 && c <= TJ && i < ni && jj < nj;
         if (own) {
             const size_t cc = cidx(i, jj, pitch);
@@ -505,6 +521,15 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
                 C.dbg[DBG_S * plane + cc] = sm.src[r - 1][c - 1];
             }
         }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int slot = 0; slot < NR1; ++slot) {
+        if (!dset[slot]) continue;
+        const int idx = tid + slot * NT;
+        const int r = idx / W1, c = idx - r * W1;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) sm.d[m][r][c] = dreg[slot][m];
     }
     __syncthreads();
 

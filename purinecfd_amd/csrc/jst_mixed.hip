@@ -309,8 +309,9 @@ template <class T> __device__ __forceinline__ void stg(T *p, unsigned q, T v)
 // Every face is evaluated once; the right n face of a lane is its neighbour's left face.
 // MIX_INNER (stages 1-4): U^(k) = U^n - a_k dt/V R in FP32 from the FP32 base dU^n.
 // MIX_LAST (stage 5): FP64 U^{n+1} = U^n - dt/V R, the new FP32 base, the next frozen dt
-// and V/dt, and the step-end partials per (8-row block, column), summed down each column in
-// row order, so they do not depend on the segment length or the batch.
+// and V/dt, and the step-end partials per (8-row block, 58-column group): summed down each
+// column in row order, then across the group's columns by a fixed butterfly, so they do not
+// depend on the segment length or the batch.
 // MIX_TERMS: export R = Fc - Fv - Fd and S of the current state (no update).
 // ---------------------------------------------------------------------------------------
 template <int MODE>
@@ -330,6 +331,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         const int ni = C.ni, nj = C.nj, pitch = C.pitch;
         const unsigned plane = (unsigned)(ni * pitch);
         const int is = w.i0, J0 = w.j0, ie = w.t;
+        const int ngroups = (nj + MIX_COLS - 1) / MIX_COLS;
         // control words: read once per item, made wave-uniform so that every pointer and
         // branch derived from them lives in scalar registers
         const int status = __builtin_amdgcn_readfirstlane(*(volatile int *)&ctl->status);
@@ -648,10 +650,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                                                          (unsigned long long)((long long)c * nj + jg));
                     }
                     if ((c & 7) == 7 || c == ie - 1) { // block end (segments start at multiples of 8)
-                        if (own) {
-                            const unsigned ch = (unsigned)((c >> 3) * nj + jg);
-                            stg(C.rpart, ch, racc);
-                            stg(C.dpart, ch, (double)dacc);
+                        // sum and minimum over the wave's columns: a fixed butterfly (every lane
+                        // ends with the same value), then one partial per (block, column group)
+                        double rs = racc;
+                        float dm = dacc;
+#pragma unroll
+                        for (int off = 32; off > 0; off >>= 1) {
+                            rs += __shfl_xor(rs, off, 64);
+                            dm = fminf(dm, __shfl_xor(dm, off, 64));
+                        }
+                        if (lane == 0) {
+                            const unsigned ch = (unsigned)((c >> 3) * ngroups + J0 / MIX_COLS);
+                            stg(C.rpart, ch, rs);
+                            stg(C.dpart, ch, (double)dm);
                         }
                         racc = 0.0;
                         dacc = __builtin_huge_valf();
@@ -708,8 +719,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 
 // ---------------------------------------------------------------------------------------
 // End of step, one block per case: residual sqrt(sum (rho^{n+1} - rho^n)^2 / NI / NJ)
-// (jst-rs solver.rs:190-200) and the next global dt from the per-(8-row block, column) partials in
-// a fixed order, totaltime += dt, step += 1, buffer rotation, and the stop decision
+// (jst-rs solver.rs:190-200) and the next global dt from the per-(8-row block, column group)
+// partials in a fixed order, totaltime += dt, step += 1, buffer rotation, and the stop decision
 // (convergence / relative drop / step limit, or the non-physical record).
 // ---------------------------------------------------------------------------------------
 template <int NT>
@@ -722,7 +733,7 @@ __global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__
     CaseCtl *ctl = C.ctl;
     const int tid = threadIdx.x;
     if (*(volatile int *)&ctl->status != JST_RUN_ACTIVE) return;
-    const int n = ((C.ni + 7) >> 3) * C.nj;
+    const int n = ((C.ni + 7) >> 3) * ((C.nj + MIX_COLS - 1) / MIX_COLS);
     double s = 0.0, m = __builtin_huge_val();
     // sequential per lane (fixed order); unrolled so that the loads are in flight together
 #pragma unroll 16
