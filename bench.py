@@ -234,7 +234,7 @@ def bench_single(args, jst):
                       "cells": cells, "event_ms_per_step": ev_ms / args.steps},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS,
-                        "traffic": None if args.mixed else profiled_traffic("stage_fp64", w["name"]),
+                        "traffic": profiled_traffic("mix_march" if args.mixed else "stage_fp64", w["name"]),
                         "stage_kernel_ms": stage_ms, "bytes_per_cell_stage": nbytes}}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w["config"], w["mesh"], w["name"])
