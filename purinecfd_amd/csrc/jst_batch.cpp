@@ -105,7 +105,7 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
-    const size_t nparts = (size_t)((ni + 7) / 8) * ((nj + MIX_COLS - 1) / MIX_COLS);
+    const size_t nparts = (size_t)((ni + 3) / 4) * ((nj + MIX_COLS - 1) / MIX_COLS);
     L.rpart = carve(nparts * 8);
     L.dpart = carve(nparts * 8);
     L.res_hist = carve((size_t)res_cap * 8);
@@ -1058,7 +1058,7 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     if (st != JST_OK) return st;
     std::unique_ptr<jst_batch> guard(b);
     {
-        const int rows = opts->segment_rows > 0 ? opts->segment_rows : 8;
+        const int rows = opts->segment_rows > 0 ? opts->segment_rows : 4;
         int v = -1;
         for (int q = 0; q < MIX_NVARIANTS; ++q)
             if (jst::MIX_SEGMENT_ROWS[q] == rows) v = q;

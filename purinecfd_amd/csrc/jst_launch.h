@@ -28,8 +28,10 @@ hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, dou
                               hipStream_t s);
 
 // FP32-mixed row-march variants: rows per work item (one wave per 58 columns x that many
-// rows); the autotuner picks one per batch.  Results do not depend on the variant.
-#define MIX_NVARIANTS 7
+// rows, a multiple of the 4-row step-end block); the autotuner picks one per batch.  Results
+// depend on the variant at FP32 rounding level only (where a segment's edge rows run the
+// general row body), never on the launch grid.
+#define MIX_NVARIANTS 8
 #define MIX_COLS 58
 extern const int MIX_SEGMENT_ROWS[MIX_NVARIANTS];
 hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
