@@ -51,6 +51,22 @@ def test_geometry_matches_golden_and_oracle_bitwise(golden, jst, oracle_lib):
     np.testing.assert_array_max_ulp(s.get("wall_dist"), o.get("wall_dist"), maxulp=1)
 
 
+@pytest.mark.parametrize("shape", [(128, 33, 0.04), (256, 129, 0.0)])
+def test_geometry_on_generated_naca_meshes_matches_oracle(jst, oracle_lib, shape):
+    """FP64 geometry precompute on stretched NACA O-meshes (wall spacing 1e-5 chord, wall
+    distance window min(max(15, NJ/5), NJ/2), geometry.rs:186-216): bit-identical to the
+    oracle except the correctly rounded sqrt of the wall distance (1 ulp)."""
+    from purinecfd_amd import meshgen
+    n_theta, rings, m = shape
+    text, _ = meshgen.naca_mesh(n_theta, rings, m, 0.4, 0.12, 1e-5, 30.0)
+    s = jst.Solver(config_text(), text)
+    o = oracle_lib.OracleSolver(config_dict(), text)
+    for name in ("vol", "cx", "cy", "tau_nx", "tau_ny", "tau_mx", "tau_my", "n_nx", "n_ny",
+                 "inv_vol"):
+        assert np.array_equal(s.get(name), o.get(name)), name
+    np.testing.assert_array_max_ulp(s.get("wall_dist"), o.get("wall_dist"), maxulp=1)
+
+
 def test_initial_state_matches_golden(golden, jst):
     doc, text = golden
     s = jst.Solver(config_text(), text)
