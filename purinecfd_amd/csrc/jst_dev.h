@@ -51,8 +51,9 @@ struct CaseDev {
     const double *tmx, *tmy;      // tau-face midpoints, (ni+1) x pitch
     const double *nnx, *nny;      // n-face normals, ni x pitch
     const double *vol, *inv_vol, *inv_wd2, *cx, *cy;
-    double *ratio;                // frozen V/dt_local per cell (jst-rs dissipation.rs:44-52)
-    double *localdt;              // frozen dt_local per cell (timestep.rs:24-47)
+    double *ratio;                // frozen V/dt_local per cell (jst-rs dissipation.rs:44-52), 2 planes
+    double *localdt;              // frozen dt_local per cell (timestep.rs:24-47), 2 planes: the
+                                  // current step reads plane (step & 1), stage 5 writes the other
     double *partial;              // step end: per-tile residual sums [ntiles] | tile min dt [ntiles]
     double *res_hist;             // residual ring buffer, res_cap entries
     int res_cap;

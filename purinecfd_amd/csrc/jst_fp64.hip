@@ -9,9 +9,9 @@
 //                   central flux (convection.rs), Green-Gauss gradient (gradient.rs),
 //                   viscous/SA diffusion (viscous.rs), JST dissipation (dissipation.rs),
 //                   SA source (source.rs), update + non-physical check (solver.rs:123-185).
-//   post_tile_kernel / post_end_kernel
-//                   end of step: L2 density residual (solver.rs:190-200) with a fixed-order
-//                   reduction, next step's local/global dt (timestep.rs:24-57), step control.
+//   post_end_kernel end of step: L2 density residual (solver.rs:190-200) from stage 5's
+//                   per-tile partials in a fixed order, next step's global dt
+//                   (timestep.rs:24-57), step control; dt_tile_kernel the dt-only refresh.
 //   forces_kernel   wall force / CL / CD (no reference counterpart).
 #include <hip/hip_runtime.h>
 
@@ -367,6 +367,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
 {
     using S = StageSmem<TI, TJ>;
     constexpr int H3 = S::H3, W3 = S::W3, H1 = S::H1, W1 = S::W1;
+    static_assert((TI + 4) * TJ >= NT && TI * (TJ + 4) >= NT, "step-end tree reuses si and sj");
     __shared__ S sm;
 
     const TileDesc tl = tiles[blockIdx.x];
@@ -408,6 +409,9 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
         }
     }
     const double dt = ctl->dt;
+    // frozen V/dt and local dt of this step: plane (step & 1); stage 5 writes the next step's
+    // pair into the other plane while neighbouring tiles still read this one
+    const size_t cur = (size_t)(step & 1) * plane, nxt = (size_t)((step + 1) & 1) * plane;
 
     // ---- phase 1: R3 load with ghost synthesis ----
     for (int idx = tid; idx < H3 * W3; idx += NT) {
@@ -446,7 +450,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
         const int r = idx / W1, c = idx - r * W1;
         int i = i0 - 1 + r;
         i = i < 0 ? 0 : (i > ni - 1 ? ni - 1 : i);
-        sm.ratio[r][c] = C.ratio[cidx(i, wrapj(j0 - 1 + c, nj), pitch)];
+        sm.ratio[r][c] = C.ratio[cur + cidx(i, wrapj(j0 - 1 + c, nj), pitch)];
     }
     __syncthreads();
 
@@ -538,6 +542,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
     // of the convective flux, the JST coefficients); the five components then follow one at
     // a time with the reference expressions, which keeps few values live at once.
     const double gamma = k.gamma, half_cfl = 0.5 * k.cfl;
+    double racc = 0.0, dmin = __builtin_huge_val(); // stage 5: this thread's step-end terms
     for (int o = tid; o < TI * TJ; o += NT) {
         const int ti = o / TJ, tj = o - ti * TJ;
         const int i = i0 + ti, j = j0 + tj;
@@ -593,9 +598,9 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
         const int vr[4] = {R1 - 1, R1 + 1, R1, R1}, vc[4] = {Q1, Q1, Q1 - 1, Q1 + 1};
 
         const size_t cc = cidx(i, j, pitch);
-        const double a_dt = k.lts ? alpha * C.localdt[cc] : alpha * dt;
+        const double a_dt = k.lts ? alpha * C.localdt[cur + cc] : alpha * dt;
         const double scl = a_dt * C.inv_vol[cc];
-        double un[4];
+        double un[4], rho_n = 0.0;
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             double fl[4], vf[4], df[4];
@@ -637,9 +642,11 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
                 // solver.rs:123-145: U = U_former - (Fc - Fv - Fd - S) * (alpha dt / V)
                 double r = fc - fv - fd;
                 if (m == 4) r -= sm.src[ti][tj];
-                const double v = former[m * plane + cc] - r * scl;
+                const double fo = former[m * plane + cc];
+                const double v = fo - r * scl;
                 dst[m * plane + cc] = v;
                 if (m < 4) un[m] = v;
+                if (m == 0) rho_n = fo;
             }
         }
         if (!TERMS) {
@@ -656,29 +663,66 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
                     ((unsigned long long)seq << 32) | (unsigned long long)((long long)i * nj + j);
                 atomicMin(&ctl->err_key, key);
             }
+            if (stage == 4) {
+                // step end, per cell: (rho^{n+1} - rho^n)^2 (solver.rs:190-200) and the next
+                // step's local dt and V/dt from U^{n+1} (timestep.rs:24-47), with the face
+                // normals this cell already holds
+                const double d = un[0] - rho_n;
+                racc += d * d;
+                const double inv = 1.0 / un[0];
+                const double vx = un[1] * inv, vy = un[2] * inv;
+                const double p = (gamma - 1.0) * (un[3] - un[0] * (vx * vx + vy * vy) * 0.5);
+                const double t = p / (k.r_gas * un[0]);
+                const double cs = sqrt(k.r_gas * k.gamma * t);
+                const double aa = 0.5 * (nx[0] + nx[1]), bb = 0.5 * (ny[0] + ny[1]);
+                const double ccv = 0.5 * (nx[2] + nx[3]), ddv = 0.5 * (ny[2] + ny[3]);
+                const double conv = fabs(vx * aa + vy * bb) + fabs(vx * ccv + vy * ddv);
+                const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(ccv * ccv + ddv * ddv));
+                const double vol = C.vol[cc];
+                const double ldt = k.cfl * vol / (conv + acou);
+                C.localdt[nxt + cc] = ldt;
+                C.ratio[nxt + cc] = vol / ldt;
+                dmin = fmin(dmin, ldt);
+            }
+        }
+    }
+    if (!TERMS && stage == 4) {
+        // the tile's residual sum and dt minimum, in the fixed tree of the step-end kernels
+        __syncthreads(); // the sensor arrays are free now: reuse them for the tree
+        double *red_s = &sm.si[0][0], *red_m = &sm.sj[0][0];
+        red_s[tid] = racc;
+        red_m[tid] = dmin;
+        __syncthreads();
+        for (int h = NT / 2; h > 0; h >>= 1) {
+            if (tid < h) {
+                red_s[tid] = red_s[tid] + red_s[tid + h];
+                red_m[tid] = fmin(red_m[tid], red_m[tid + h]);
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            C.partial[tl.t] = red_s[0];
+            C.partial[C.ntiles + tl.t] = red_m[0];
         }
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// End of step (mode 1) or dt-only refresh (mode 0), in two launches.
-//   post_tile_kernel, one block per tile:
-//     both:   localdt and V/dt of the new state (timestep.rs:24-47) per cell; the tile's
-//             minimum dt into partial[ntiles + t];
-//     mode 1: the tile's sum of (rho^{n+1}-rho^n)^2 (solver.rs:190-200) into partial[t].
-//   post_end_kernel, one block per case:
-//     mode 1: residual sqrt(sum / NI / NJ), the tile sums added in a fixed order (so the
-//             value does not depend on scheduling), totaltime += dt, step += 1, rotation
-//             advance, convergence / stop control;
-//     both:   dt of the next step = minimum over the tiles (exact, order-free).
-// Splitting the end off keeps device-scope fences and atomics out of the per-tile blocks.
+// End of step.  Stage 5 leaves per tile the sum of (rho^{n+1}-rho^n)^2 (solver.rs:190-200) in
+// partial[t] and the minimum of the next local dt in partial[ntiles + t], and writes the next
+// step's local dt and V/dt (timestep.rs:24-47).  post_end_kernel (one block per case) then
+// adds the tile sums in a fixed order (so the value does not depend on scheduling):
+//   mode 1: residual sqrt(sum / NI / NJ), totaltime += dt, step += 1, rotation advance,
+//           convergence / stop control;
+//   both:   dt of the next step = minimum over the tiles (exact, order-free).
+// dt_tile_kernel is the dt-only refresh of the current state (mode 0: at creation, after a
+// state injection, before a residual-terms export): local dt and V/dt into the current plane
+// and the tile minima.
 // ---------------------------------------------------------------------------------------
 template <int TI, int TJ, int NT>
-__global__ __launch_bounds__(NT) void post_tile_kernel(const CaseDev *__restrict__ cases,
-                                                       const TileDesc *__restrict__ tiles,
-                                                       int mode)
+__global__ __launch_bounds__(NT) void dt_tile_kernel(const CaseDev *__restrict__ cases,
+                                                     const TileDesc *__restrict__ tiles)
 {
-    __shared__ double red_s[NT];
     __shared__ double red_m[NT];
     const TileDesc tl = tiles[blockIdx.x];
     const CaseDev &C = cases[tl.c];
@@ -687,25 +731,18 @@ __global__ __launch_bounds__(NT) void post_tile_kernel(const CaseDev *__restrict
     const int ni = C.ni, nj = C.nj, pitch = C.pitch;
     const size_t plane = (size_t)ni * pitch;
     const int tid = threadIdx.x;
-    const int status = *(volatile const int *)&ctl->status;
-    const unsigned long long ek = *(volatile const unsigned long long *)&ctl->err_key;
     const int rot = *(volatile const int *)&ctl->rot;
-    if (mode == 1 && (status != JST_RUN_ACTIVE || ek != JST_ERR_NONE)) return;
-    const int b1 = rot == 2 ? 0 : rot + 1;
-    const double *unew = mode == 1 ? C.ubuf[b1] : C.ubuf[rot];
-    const double *uold = C.ubuf[rot];
-    double acc = 0.0, dmin = __builtin_huge_val();
+    const long long step = *(volatile const long long *)&ctl->step;
+    const size_t cur = (size_t)(step & 1) * plane;
+    const double *u_now = C.ubuf[rot];
+    double dmin = __builtin_huge_val();
     for (int o = tid; o < TI * TJ; o += NT) {
         const int ti = o / TJ, tj = o - ti * TJ;
         const int i = tl.i0 + ti, j = tl.j0 + tj;
         if (i >= ni || j >= nj) continue;
         const size_t c = cidx(i, j, pitch);
         double u[5];
-        load_u(unew, plane, c, u);
-        if (mode == 1) {
-            const double d = u[0] - uold[c];
-            acc += d * d;
-        }
+        load_u(u_now, plane, c, u);
         Prim q;
         unpack(k, u, q);
         const double cs = sqrt(k.r_gas * k.gamma * q.t);
@@ -718,22 +755,18 @@ __global__ __launch_bounds__(NT) void post_tile_kernel(const CaseDev *__restrict
         const double conv = fabs(q.vx * aa + q.vy * bb) + fabs(q.vx * ccv + q.vy * ddv);
         const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(ccv * ccv + ddv * ddv));
         const double ldt = k.cfl * C.vol[c] / (conv + acou);
-        C.localdt[c] = ldt;
-        C.ratio[c] = C.vol[c] / ldt;
+        C.localdt[cur + c] = ldt;
+        C.ratio[cur + c] = C.vol[c] / ldt;
         dmin = fmin(dmin, ldt);
     }
-    red_s[tid] = acc;
     red_m[tid] = dmin;
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            red_s[tid] = red_s[tid] + red_s[tid + s];
-            red_m[tid] = fmin(red_m[tid], red_m[tid + s]);
-        }
+        if (tid < s) red_m[tid] = fmin(red_m[tid], red_m[tid + s]);
         __syncthreads();
     }
     if (tid == 0) {
-        C.partial[tl.t] = red_s[0];
+        C.partial[tl.t] = 0.0;
         C.partial[C.ntiles + tl.t] = red_m[0];
     }
 }
@@ -902,11 +935,14 @@ hipError_t launch_stage_fp64(const CaseDev *cases, const TileDesc *tiles, int nt
     return hipGetLastError();
 }
 
+// mode 1: end of step (stage 5 has left the tile partials); mode 0: dt refresh of the current
+// state.
 hipError_t launch_post_fp64(const CaseDev *cases, const TileDesc *tiles, int ntiles, int mode,
                             hipStream_t s)
 {
-    hipLaunchKernelGGL((post_tile_kernel<FP64_TI, FP64_TJ, FP64_NT>), dim3(ntiles), dim3(FP64_NT),
-                       0, s, cases, tiles, mode);
+    if (mode == 0)
+        hipLaunchKernelGGL((dt_tile_kernel<FP64_TI, FP64_TJ, FP64_NT>), dim3(ntiles), dim3(FP64_NT),
+                           0, s, cases, tiles);
     hipLaunchKernelGGL((post_end_kernel<256>), dim3(1), dim3(256), 0, s, cases, mode);
     return hipGetLastError();
 }

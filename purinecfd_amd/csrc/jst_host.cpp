@@ -869,8 +869,8 @@ jst_status jst_create(const char *config_json, const char *mesh_text, jst_precis
     ALLOC(c.g.nny, c.plane);
     ALLOC(c.g.nmx, c.plane);
     ALLOC(c.g.nmy, c.plane);
-    ALLOC(c.ratio, c.plane);
-    ALLOC(c.localdt, c.plane);
+    ALLOC(c.ratio, 2 * c.plane);   // two planes: this step's (step & 1) and the next one's
+    ALLOC(c.localdt, 2 * c.plane);
     ALLOC(c.res_hist, (size_t)c.res_cap);
     ALLOC(c.scratch, 4 * (size_t)c.nj + 2); // wall-force terms [4][nj] + {Fx, Fy}
     auto tiles = make_tiles(0, c.ni, c.nj, FP64_TI, FP64_TJ);
@@ -1121,8 +1121,16 @@ jst_status jst_get_field(jst_ctx *x, const char *name, double *dst, size_t n)
                {"tau_nx", x->g.tnx, true},       {"tau_ny", x->g.tny, true},
                {"tau_mx", x->g.tmx, true},       {"tau_my", x->g.tmy, true},
                {"n_nx", x->g.nnx, false},        {"n_ny", x->g.nny, false},
-               {"n_mx", x->g.nmx, false},        {"n_my", x->g.nmy, false},
-               {"localdt", x->localdt, false},   {"ratio", x->ratio, false}};
+               {"n_mx", x->g.nmx, false},        {"n_my", x->g.nmy, false}};
+    if (s == "localdt" || s == "ratio") {
+        // the frozen pair of the current step: plane (step & 1)
+        if (n < ncell) return fail(JST_ERR_ARG, "destination too small");
+        if ((st = ctl_fetch(x)) != JST_OK) return st;
+        const double *base = (s == "ratio" ? x->ratio : x->localdt) + (size_t)(x->h_ctl->step & 1) * x->plane;
+        if ((st = download(x, base, x->plane, h)) != JST_OK) return st;
+        gather(h, 0, ni);
+        return JST_OK;
+    }
     for (auto &gd : geo) {
         if (s == gd.n) {
             int rows = gd.face ? ni + 1 : ni;
