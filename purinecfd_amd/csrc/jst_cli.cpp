@@ -1,6 +1,7 @@
 // jst-gpu — command-line driver with the flags and outputs of jst-rs's CLI
 // (jst-rs/src/bin/jst.rs:10-123): --mesh --config --steps --out --reslog --quiet --threads
-// --no-output, plus --precision fp64|mixed and --wall PATH (wall Cp/Cf table).  stdout lines, res.log and result.csv use the
+// --no-output, plus --precision fp64|mixed, --wall PATH (wall Cp/Cf table) and --forces
+// (CL and CD of the final state).  stdout lines, res.log and result.csv use the
 // same text formats (Rust `{:6}`, `{:.6e}`, `{:.8e}`), so scripts that read the CPU
 // solver's output read this one's.  Built on the public C ABI only (include/jst_gpu.h).
 #include <algorithm>
@@ -33,7 +34,7 @@ struct Cli {
     std::string mesh = "fangdata.txt", config = "config.json", out = "result.csv", reslog = "res.log";
     std::string wall; // empty: no wall table
     long steps = 0;
-    bool quiet = false, no_output = false;
+    bool quiet = false, no_output = false, forces = false;
     long threads = 0;
     jst_precision prec = JST_FP64;
 };
@@ -44,7 +45,7 @@ int usage(const char *msg)
     fprintf(stderr,
             "Usage: jst-gpu [--mesh PATH] [--config PATH] [--steps N] [--out PATH] [--reslog PATH]\n"
             "               [--quiet] [--threads N] [--no-output] [--precision fp64|mixed]\n"
-            "               [--wall PATH]\n");
+            "               [--wall PATH] [--forces]\n");
     return 2;
 }
 
@@ -83,6 +84,7 @@ int main(int argc, char **argv)
         else if (s == "--threads") cli.threads = atol(val("--threads"));
         else if (s == "--no-output") cli.no_output = true;
         else if (s == "--wall") cli.wall = val("--wall");
+        else if (s == "--forces") cli.forces = true;
         else if (s == "--precision") {
             std::string p = val("--precision");
             if (p == "fp64") cli.prec = JST_FP64;
@@ -150,6 +152,15 @@ int main(int argc, char **argv)
             return 1;
         }
         printf("all data is written in %s\n", cli.out.c_str());
+    }
+    if (cli.forces) {
+        double cl = 0.0, cd = 0.0;
+        if (jst_forces(ctx, nullptr, nullptr, &cl, &cd) != JST_OK) {
+            fprintf(stderr, "Error: %s\n", jst_last_error());
+            jst_destroy(ctx);
+            return 1;
+        }
+        printf("CL: %s  CD: %s\n", rust_exp(cl, 8).c_str(), rust_exp(cd, 8).c_str());
     }
     if (!cli.wall.empty()) {
         if (jst_write_wall_csv(ctx, cli.wall.c_str()) != JST_OK) {

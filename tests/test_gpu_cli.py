@@ -90,6 +90,21 @@ def test_cli_wall_table_equals_the_solver_output(tmp_path, jst, precision):
         assert np.allclose(got[:, c], w[k], rtol=1e-8, atol=0.0), k
 
 
+def test_cli_forces_line_equals_the_solver_forces(tmp_path, jst):
+    rc, out, err = run_cli(tmp_path, "--mesh", str(GOLDEN / "fangdata.txt"), "--config",
+                           str(GOLDEN / "config.json"), "--steps", "9", "--quiet", "--forces",
+                           "--no-output")
+    assert rc == 0, err
+    line = out.splitlines()[-1]
+    m = re.fullmatch(r"CL: (-?\d\.\d{8}e-?\d+)  CD: (-?\d\.\d{8}e-?\d+)", line)
+    assert m, line
+    s = jst.Solver((GOLDEN / "config.json").read_text(), mesh_text("fangdata.txt"))
+    s.run(9)
+    f = s.forces()
+    assert abs(float(m.group(1)) - f["cl"]) <= 5e-9 * max(1e-12, abs(f["cl"]))
+    assert abs(float(m.group(2)) - f["cd"]) <= 5e-9 * max(1e-12, abs(f["cd"]))
+
+
 def test_cli_reports_nonphysical(tmp_path):
     cfg = config_dict()
     cfg["simulation"]["CFL"] = 200.0

@@ -204,6 +204,23 @@ def test_meshes_of_several_shapes_stay_physical(jst, oracle_lib):
         field_cmp(s.get("nut"), o.get("nut"), f"nut {rings}x{nj}", 1e-9)
 
 
+@pytest.mark.parametrize("rings,nj,stretch", [(4, 8, 1.0), (4, 9, 1.3), (5, 61, 1.2)])
+def test_smallest_meshes_match_oracle(jst, oracle_lib, rings, nj, stretch):
+    """The smallest mesh Mesh::parse accepts (4 rings x 8 points, mesh.rs:159-165), an odd ring
+    length, and a ring three points wider than one 58-column wave group of the mixed path:
+    FP64 at 1e-10 of the oracle after 60 steps, FP32-mixed within 1e-5 of it."""
+    text = ellipse_mesh(rings, nj, stretch)
+    o = oracle_lib.OracleSolver(config_dict(), text)
+    o.run(60)
+    s64 = jst.Solver(config_text(), text, jst.FP64)
+    smx = jst.Solver(config_text(), text, jst.MIXED)
+    assert s64.run(60).steps == 60 and smx.run(60).steps == 60
+    for name in ("rho", "u", "v", "p", "nut"):
+        field_cmp(s64.get(name), o.get(name), f"fp64 {name}", 1e-10, 1e-300)
+    for name in ("rho", "u", "v", "p"):
+        field_cmp(smx.get(name), o.get(name), f"mixed {name}", 1e-5, 1e-300)
+
+
 def test_converges_and_stops_on_the_exact_step(jst, oracle_lib):
     """properties.rs:178-201: the device stops at the same step as the reference loop."""
     text = ellipse_mesh(9, 32, 1.0)
