@@ -774,6 +774,9 @@ jst_status jst_batch_forces(jst_batch *b, int case_id, double *cl, double *cd)
     return JST_OK;
 }
 
+// The frozen dt, local dt and V/dt of the current state are always current on this path (set
+// at admission, then by stage 5 of every step), so the export evaluates the terms with them
+// and leaves the trajectory untouched.
 jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out)
 {
     HostCase *hc = get_case(b, case_id);
@@ -782,11 +785,10 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out)
     case_items(*hc, jst::MIX_SEGMENT_ROWS[b->variant], t);
     TileDesc *dt = nullptr;
     HIPCHK(hipMalloc(&dt, t.size() * sizeof(TileDesc)));
+    std::unique_ptr<TileDesc, void (*)(TileDesc *)> guard(dt, [](TileDesc *p) { (void)hipFree(p); });
     HIPCHK(hipMemcpyAsync(dt, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice, b->stream));
-    HIPCHK(jst::launch_dt_mixed(b->d_cases, hc->d_posts, hc->npost, b->stream));
     HIPCHK(jst::launch_stage_mixed(b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true, b->stream));
-    jst_status st = fetch_ctl(b);
-    HIPCHK(hipFree(dt));
+    jst_status st = fetch_ctl(b); // synchronises the stream before the list is freed
     if (st != JST_OK) return st;
     if (dt_out) *dt_out = b->h_ctl[case_id].dt;
     return JST_OK;

@@ -54,6 +54,23 @@ def test_mixed_residual_matches_oracle_fp32_level(jst, oracle_lib):
     print(f"mixed residual worst field-normed error vs oracle: {max(worst, err):.2e}")
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_residual_terms_export_leaves_the_run_unchanged(jst, precision):
+    """Solver::residual_terms is read-only (solver.rs:86-94): a run with an export in the
+    middle ends bitwise where the same run without it does."""
+    prec = jst.FP64 if precision == "fp64" else jst.MIXED
+    text = mesh_text("cyl_17x40.txt")
+    a = jst.Solver(config_text(), text, prec)
+    b = jst.Solver(config_text(), text, prec)
+    a.run(10)
+    a.residual_terms()
+    a.run(10)
+    b.run(20)
+    for n in ("rho", "u", "v", "p", "nut"):
+        assert np.array_equal(a.get(n), b.get(n)), n
+    assert a.residual() == b.residual()
+
+
 def test_mixed_cylinder_c1_cl_cd_within_1e4(jst, oracle_lib):
     """Cylinder case (40 x 17 points, M 0.2, Re 1e3, CFL 2), 2000 iterations: mixed vs FP64
     oracle |dCL|,|dCD| <= 1e-4."""
