@@ -1053,14 +1053,7 @@ jst_status jst_advance(jst_ctx *x, int nsteps, double *res_out, jst_nonphysical 
     // advance never stops on convergence: the reference's advance() has no stop criterion
     jst_status st = ctl_set(x, &CaseCtl::timing, 1);
     if (st != JST_OK) return st;
-    st = jst_run(
-        x, nsteps,
-        [](long, double r, void *u) {
-            Acc *a = (Acc *)u;
-            if (a->out) a->out[a->n] = r;
-            a->n++;
-        },
-        &acc, &rep, err);
+    st = jst_run(x, nsteps, acb, &acc, &rep, err);
     jst_status st2 = ctl_set(x, &CaseCtl::timing, 0);
     return st != JST_OK ? st : st2;
 }
