@@ -373,6 +373,21 @@ template <class T> jst_status set_ctl_field(jst_batch *b, int slot, T CaseCtl::*
     return JST_OK;
 }
 
+// The same field of several cases: the copies are queued together and waited for once.
+template <class T>
+jst_status set_ctl_fields(jst_batch *b, const std::vector<int> &slots, T CaseCtl::*field, T value)
+{
+    for (int slot : slots) {
+        CaseCtl *h = &b->h_ctl[slot];
+        h->*field = value;
+        const size_t off = (size_t)((char *)&(h->*field) - (char *)h);
+        HIPCHK(hipMemcpyAsync((char *)&b->d_ctl[slot] + off, &(h->*field), sizeof(T),
+                              hipMemcpyHostToDevice, b->stream));
+    }
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return JST_OK;
+}
+
 HostCase *get_case(const jst_batch *b, int id)
 {
     if (!b || id < 0 || id >= (int)b->cases.size()) return nullptr;
@@ -885,10 +900,8 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
     if (st != JST_OK) return st;
     std::vector<int> touched;
     for (auto &hc : b->cases)
-        if (hc && b->h_ctl[hc->slot].status == JST_RUN_ACTIVE) {
-            if ((st = set_ctl_field(b, hc->slot, &CaseCtl::timing, 1)) != JST_OK) return st;
-            touched.push_back(hc->slot);
-        }
+        if (hc && b->h_ctl[hc->slot].status == JST_RUN_ACTIVE) touched.push_back(hc->slot);
+    if ((st = set_ctl_fields(b, touched, &CaseCtl::timing, 1)) != JST_OK) return st;
     if (b->dirty && (st = rebuild_lists(b)) != JST_OK) return st;
     // build (and warm) the graphs outside the timed region
     if ((st = prepare_graphs(b, nsteps)) != JST_OK) return st;
@@ -922,8 +935,7 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
         for (auto &e : evs) (void)hipEventDestroy(e);
         *stage_ms = sum / (5.0 * nsteps);
     }
-    for (int slot : touched)
-        if ((st = set_ctl_field(b, slot, &CaseCtl::timing, 0)) != JST_OK) return st;
+    if ((st = set_ctl_fields(b, touched, &CaseCtl::timing, 0)) != JST_OK) return st;
     if ((st = fetch_ctl(b)) != JST_OK) return st;
     // a diverged case stops its own kernels early: a timing that includes one is not valid
     for (int slot : touched)
@@ -945,10 +957,8 @@ jst_status jst_batch_time_kernels(jst_batch *b, int nsteps, double *inner_ms, do
     if (st != JST_OK) return st;
     std::vector<int> touched;
     for (auto &hc : b->cases)
-        if (hc && b->h_ctl[hc->slot].status == JST_RUN_ACTIVE) {
-            if ((st = set_ctl_field(b, hc->slot, &CaseCtl::timing, 1)) != JST_OK) return st;
-            touched.push_back(hc->slot);
-        }
+        if (hc && b->h_ctl[hc->slot].status == JST_RUN_ACTIVE) touched.push_back(hc->slot);
+    if ((st = set_ctl_fields(b, touched, &CaseCtl::timing, 1)) != JST_OK) return st;
     if (b->dirty && (st = rebuild_lists(b)) != JST_OK) return st;
     std::vector<hipEvent_t> evs(12 * (size_t)nsteps);
     for (auto &e : evs) HIPCHK(hipEventCreate(&e));
@@ -975,8 +985,7 @@ jst_status jst_batch_time_kernels(jst_batch *b, int nsteps, double *inner_ms, do
     if (inner_ms) *inner_ms = acc[0] / (4.0 * nsteps);
     if (last_ms) *last_ms = acc[1] / nsteps;
     if (end_ms) *end_ms = acc[2] / nsteps;
-    for (int slot : touched)
-        if ((st = set_ctl_field(b, slot, &CaseCtl::timing, 0)) != JST_OK) return st;
+    if ((st = set_ctl_fields(b, touched, &CaseCtl::timing, 0)) != JST_OK) return st;
     if ((st = fetch_ctl(b)) != JST_OK) return st;
     for (int slot : touched)
         if (b->h_ctl[slot].err_key != JST_ERR_NONE)
