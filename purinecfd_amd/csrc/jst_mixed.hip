@@ -239,13 +239,15 @@ __device__ __forceinline__ float sensor32(float pinf4, float dm, float d0, float
 // 61..63 carry the periodic halo (the widest stencil, a face's sensors, reaches 3 columns
 // out).  Values of the j-neighbours move between lanes with DPP wave shifts, which read
 // every lane, so they are only ever called from wave-uniform control flow.
+// bound_ctrl: the edge lane with no source (lane 0, resp. 63: both halo lanes) reads 0, so
+// the destination needs no initial value (no extra move per shift).
 __device__ __forceinline__ float from_left(float x) // lane l receives lane l-1
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ float from_right(float x) // lane l receives lane l+1
 {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0x130, 0xf, 0xf, true));
 }
 
 // One face: perturbation Euler flux of the face average, JST dissipation with the 4-cell
