@@ -1088,6 +1088,7 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     std::vector<long> last_step(opts->max_cases, 0);
     double cell_stages = 0.0;
     int next = 0, done = 0, nonphys = 0, maxc = 0, active = 0, tuned_for = -1;
+    double step_s = 0.0; // measured wall time per step of the current mix (round average)
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
@@ -1127,7 +1128,25 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
             if ((st = tune_impl(b, 3, false, &v, &g)) != JST_OK) return st;
             tuned_for = active;
         }
-        if ((st = jst_batch_step(b, spr, 1)) != JST_OK) return st;
+        // Round length: at most steps_per_round steps, ending at the nearest step cap of a
+        // running case or at the next arrival (from the measured step time), so that neither
+        // a retirement nor an admission waits for the rest of a long round.  Results do not
+        // depend on it: every case advances by whole steps on its own data.
+        int nsteps = spr;
+        for (int id = 0; id < opts->max_cases; ++id)
+            if (slot_req[id] >= 0) {
+                const long left = get_case(b, id)->max_steps - last_step[id];
+                if (left > 0 && left < nsteps) nsteps = (int)left;
+            }
+        if (next < n && step_s > 0.0) {
+            const double wait = std::max(reqs[order[next]].arrival_s - now(), 0.0);
+            const double k = std::ceil(wait / step_s);
+            if (k < (double)nsteps) nsteps = std::max(1, (int)k);
+        }
+        const double t_round = now();
+        if ((st = jst_batch_step(b, nsteps, 1)) != JST_OK) return st;
+        const double per = (now() - t_round) / nsteps;
+        step_s = step_s > 0.0 ? 0.5 * (step_s + per) : per;
         // account and retire
         bool progressed = false;
         for (int id = 0; id < opts->max_cases; ++id) {

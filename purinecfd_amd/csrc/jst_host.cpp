@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <charconv>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -386,50 +387,70 @@ std::vector<std::string> split_ws(const std::string &line)
     return out;
 }
 
+// Whitespace of the mesh text format (the ASCII set of Rust's str::trim / split_whitespace).
+inline bool mesh_ws(char c)
+{
+    return c == ' ' || c == '\t' || c == '\r' || c == '\n' || c == '\v' || c == '\f';
+}
+
+// parse_f64 on the token [a, b) without copying it: std::from_chars (correctly rounded, as
+// strtod) is the fast path; what it declines — a leading '+', values out of the double range
+// (Rust parses those to +-inf or 0) — goes through parse_f64, so every token gets exactly
+// the value and verdict parse_f64 gives it.
+bool parse_f64_view(const char *a, const char *b, double &out)
+{
+    if (a == b) return false;
+    for (const char *q = a; q < b; ++q)
+        if (*q == 'x' || *q == 'X' || *q == '(' || *q == ')' || *q == 'p' || *q == 'P') return false;
+    const auto r = std::from_chars(a, b, out);
+    if (r.ec == std::errc() && r.ptr == b) return true;
+    return parse_f64(std::string(a, b), out);
+}
+
 jst_status parse_mesh(const char *text, Mesh &m)
 {
     if (!text) return fail(JST_ERR_ARG, "mesh text is NULL");
-    std::string all(text);
-    if (all.empty()) return fail(JST_ERR_EMPTY, "mesh file contains no points");
-    std::vector<std::string> lines;
-    {
-        size_t s = 0;
-        for (;;) {
-            size_t nl = all.find('\n', s);
-            std::string ln = all.substr(s, nl == std::string::npos ? std::string::npos : nl - s);
-            if (!ln.empty() && ln.back() == '\r') ln.pop_back();
-            lines.push_back(ln);
-            if (nl == std::string::npos) break;
-            s = nl + 1;
-            if (s >= all.size()) break;
-        }
-    }
-    auto trim = [](const std::string &s) {
-        size_t a = s.find_first_not_of(" \t\r\n\v\f"), b = s.find_last_not_of(" \t\r\n\v\f");
-        return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
-    };
-    std::string header = trim(lines[0]);
-    auto ht = split_ws(header);
+    const char *const end = text + strlen(text);
+    if (text == end) return fail(JST_ERR_EMPTY, "mesh file contains no points");
+    // lines split at '\n' and trimmed in place (a mesh of 513 x 257 points is 5 MB of text)
     long ni = 0, nj = 0;
-    if (ht.size() < 2 || !parse_usize(ht[0], ni) || !parse_usize(ht[1], nj)) {
-        char buf[512];
-        snprintf(buf, sizeof buf, "bad header \"%s\": expected `i_total j_total`", header.c_str());
-        return fail(JST_ERR_BAD_HEADER, buf);
-    }
     std::vector<double> xy;
-    xy.reserve((size_t)std::max(0L, ni * nj) * 2);
-    for (size_t k = 1; k < lines.size(); ++k) {
-        std::string t = trim(lines[k]);
-        if (t.empty()) continue;
-        auto tok = split_ws(t);
-        double x, y;
-        if (tok.size() < 2 || !parse_f64(tok[0], x) || !parse_f64(tok[1], y)) {
-            char buf[512];
-            snprintf(buf, sizeof buf, "line %zu: cannot parse point \"%s\"", k + 1, t.c_str());
-            return fail(JST_ERR_BAD_POINT, buf);
+    size_t k = 0; // line index, 0 = header
+    for (const char *s = text;;) {
+        const char *nl = (const char *)memchr(s, '\n', (size_t)(end - s));
+        const char *a = s, *b = nl ? nl : end;
+        while (a < b && mesh_ws(*a)) ++a;
+        while (b > a && mesh_ws(b[-1])) --b;
+        if (k == 0) {
+            const std::string header(a, b);
+            auto ht = split_ws(header);
+            if (ht.size() < 2 || !parse_usize(ht[0], ni) || !parse_usize(ht[1], nj)) {
+                char buf[512];
+                snprintf(buf, sizeof buf, "bad header \"%s\": expected `i_total j_total`", header.c_str());
+                return fail(JST_ERR_BAD_HEADER, buf);
+            }
+            xy.reserve((size_t)std::max(0L, ni * nj) * 2);
+        } else if (a < b) {
+            const char *e0 = a;
+            while (e0 < b && !mesh_ws(*e0)) ++e0;
+            const char *a1 = e0;
+            while (a1 < b && mesh_ws(*a1)) ++a1;
+            const char *e1 = a1;
+            while (e1 < b && !mesh_ws(*e1)) ++e1;
+            double x, y;
+            if (a1 == b || !parse_f64_view(a, e0, x) || !parse_f64_view(a1, e1, y)) {
+                char buf[512];
+                snprintf(buf, sizeof buf, "line %zu: cannot parse point \"%s\"", k + 1,
+                         std::string(a, b).c_str());
+                return fail(JST_ERR_BAD_POINT, buf);
+            }
+            xy.push_back(x);
+            xy.push_back(y);
         }
-        xy.push_back(x);
-        xy.push_back(y);
+        ++k;
+        if (!nl) break;
+        s = nl + 1;
+        if (s >= end) break;
     }
     long got = (long)(xy.size() / 2);
     if (got == 0) return fail(JST_ERR_EMPTY, "mesh file contains no points");

@@ -137,6 +137,37 @@ def test_mesh_parse_matches_oracle(jst, oracle_lib):
         assert np.array_equal(m.xy, xy)
 
 
+def test_mesh_number_spellings(jst, oracle_lib):
+    # the spellings Rust's f64::from_str takes (mesh.rs parses points with it): a leading '+',
+    # no digits before or after the point, out-of-range exponents (to +-inf and 0), subnormals,
+    # inf/NaN words, and any ASCII whitespace between and around the two numbers; Python's
+    # float() reads every one of them to the same correctly rounded value
+    odd = ["+1.5", ".5", "1.", "-0.0", "1e400", "-1e400", "1e-400", "4.9e-324",
+           "2.2250738585072011e-308", "179769313486231570000e288", "inf", "-Infinity", "NaN",
+           "0.1000000000000000055511151231257827", "123456789012345678901234567890", "7E+2"]
+    lines, want = ["4 8"], []
+    for k in range(32):
+        # ring 1 (points 8..15) holds the odd spellings, two per point
+        x, y = (odd[2 * (k - 8)], odd[2 * (k - 8) + 1]) if 8 <= k < 16 else (repr(float(k)), repr(-float(k)))
+        sep = ("\t", " \v ", "\f", "  ")[k % 4]
+        lines.append(f" {x}{sep}{y}  extra" if k == 5 else f"{sep}{x}{sep}{y}\r")
+        want.append((float(x), float(y)))
+    text = "\n".join(lines) + "\n\n"
+    # ring 0 starts and ends with distinct ordinary points: the closing-duplicate trim is not
+    # in play
+    want = np.array(want)
+    m = jst.Mesh.parse(text)
+    assert (m.n_rings, m.n_theta) == (4, 8)
+    assert np.array_equal(m.xy, want, equal_nan=True)
+    assert np.array_equal(np.signbit(m.xy), np.signbit(want))
+    r, t, xy = oracle_lib.parse_mesh(text)
+    assert np.array_equal(xy, want, equal_nan=True)
+    for bad in ("0x1p3", "1e", "1.5f", "-", "1..2", "nan(1)", "1_0"):
+        with pytest.raises(jst.MeshError) as e:
+            jst.Mesh.parse(text.replace("\f2.0\f", f"\f{bad}\f", 1))
+        assert e.value.kind == "BadPoint" and "line 4" in str(e.value), bad
+
+
 def test_repository_mesh(jst):
     m = jst.Mesh.parse(mesh_text("fangdata.txt"))
     assert (m.n_rings, m.n_theta, m.n_cells()) == (10, 12, 108)
