@@ -100,7 +100,9 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     for (int q = 0; q < 6; ++q) L.g64[q] = carve(plane * 8);
     for (int q = 6; q < 10; ++q) L.g64[q] = carve(fplane * 8);
     for (int q = 10; q < 14; ++q) L.g64[q] = carve(plane * 8);
-    // FP32 geometry: tnx tny (fplane) | nnx nny vol inv_vol inv_wd2 (plane)
+    // FP32 geometry: tnx tny (fplane) | nnx nny vol inv_vol inv_wd2 (plane), then V/dt and
+    // the local dt.  The stage kernels reach all of them through one buffer descriptor that
+    // starts at tnx and ends with localdt[1]: keep tnx first and localdt[1] last.
     for (int q = 0; q < 2; ++q) L.g32[q] = carve(fplane * 4);
     for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
@@ -592,7 +594,9 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     if (hc->cfg.halo != JST_H) return fail(JST_ERR_CONFIG, "unsupported halo: kernels are built for IM=3");
     if (mesh.n_rings - 1 < JST_H || mesh.n_theta < 4)
         return fail(JST_ERR_ARG, "the mixed path needs at least 3 cell rings and 4 cells per ring");
-    // every array of a case (at most 6 planes of (NI+1) x pitch, 8 bytes) below 4 GiB
+    // 32-bit byte offsets: every array of a case (at most 6 planes of (NI+1) x pitch, 8
+    // bytes), a 5-plane FP64 state (40 B per cell) and the FP32 geometry / V/dt / local dt
+    // span the stage kernels address through one descriptor (44 B per cell) stay below 4 GiB
     if ((size_t)mesh.n_rings * ((mesh.n_theta + 31) / 32 * 32) * 6 * 8 >= ((size_t)1 << 32))
         return fail(JST_ERR_ARG, "mesh too large for 32-bit array offsets");
     int slot = -1;
