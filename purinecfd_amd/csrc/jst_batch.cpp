@@ -107,7 +107,7 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
-    const size_t nparts = (size_t)((ni + 3) / 4) * ((nj + MIX_COLS - 1) / MIX_COLS);
+    const size_t nparts = (size_t)((ni + 1) / 2) * ((nj + MIX_COLS - 1) / MIX_COLS); // 2-row blocks at most
     L.rpart = carve(nparts * 8);
     L.dpart = carve(nparts * 8);
     L.res_hist = carve((size_t)res_cap * 8);
@@ -245,6 +245,10 @@ jst_status drop_graph(jst_batch *b)
     return JST_OK;
 }
 
+// log2 of the rows per step-end partial: 4-row blocks, 2-row blocks under 2-row segments (a
+// segment covers whole blocks, so that each partial has one writer)
+int block_shift(const jst_batch *b) { return jst::MIX_SEGMENT_ROWS[b->variant] % 4 == 0 ? 2 : 1; }
+
 // Work items of one case for the row march: 58-column groups x row segments of `rows` rows.
 void case_items(const HostCase &hc, int rows, std::vector<TileDesc> &out)
 {
@@ -299,9 +303,9 @@ jst_status rebuild_lists(jst_batch *b)
 jst_status launch_one_step(jst_batch *b)
 {
     for (int k = 0; k < 5; ++k)
-        HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k], false,
+        HIPCHK(jst::launch_stage_mixed(block_shift(b), b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k], false,
                                        b->stream));
-    HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
+    HIPCHK(jst::launch_end_mixed(block_shift(b), b->d_cases, b->d_slots, b->nslots, b->stream));
     return JST_OK;
 }
 
@@ -806,7 +810,7 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out)
     HIPCHK(hipMalloc(&dt, t.size() * sizeof(TileDesc)));
     std::unique_ptr<TileDesc, void (*)(TileDesc *)> guard(dt, [](TileDesc *p) { (void)hipFree(p); });
     HIPCHK(hipMemcpyAsync(dt, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice, b->stream));
-    HIPCHK(jst::launch_stage_mixed(b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true, b->stream));
+    HIPCHK(jst::launch_stage_mixed(block_shift(b), b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true, b->stream));
     jst_status st = fetch_ctl(b); // synchronises the stream before the list is freed
     if (st != JST_OK) return st;
     if (dt_out) *dt_out = b->h_ctl[case_id].dt;
@@ -923,11 +927,11 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
         for (int s = 0; s < nsteps; ++s) {
             for (int k = 0; k < 5; ++k) {
                 HIPCHK(hipEventRecord(evs[10 * s + 2 * k], b->stream));
-                HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
+                HIPCHK(jst::launch_stage_mixed(block_shift(b), b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
                                                false, b->stream));
                 HIPCHK(hipEventRecord(evs[10 * s + 2 * k + 1], b->stream));
             }
-            HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
+            HIPCHK(jst::launch_end_mixed(block_shift(b), b->d_cases, b->d_slots, b->nslots, b->stream));
         }
         HIPCHK(hipStreamSynchronize(b->stream));
         double sum = 0.0;
@@ -969,12 +973,12 @@ jst_status jst_batch_time_kernels(jst_batch *b, int nsteps, double *inner_ms, do
     for (int s = 0; s < nsteps; ++s) {
         for (int k = 0; k < 5; ++k) {
             HIPCHK(hipEventRecord(evs[12 * s + 2 * k], b->stream));
-            HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
+            HIPCHK(jst::launch_stage_mixed(block_shift(b), b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
                                            false, b->stream));
             HIPCHK(hipEventRecord(evs[12 * s + 2 * k + 1], b->stream));
         }
         HIPCHK(hipEventRecord(evs[12 * s + 10], b->stream));
-        HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
+        HIPCHK(jst::launch_end_mixed(block_shift(b), b->d_cases, b->d_slots, b->nslots, b->stream));
         HIPCHK(hipEventRecord(evs[12 * s + 11], b->stream));
     }
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -1073,7 +1077,7 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     if (st != JST_OK) return st;
     std::unique_ptr<jst_batch> guard(b);
     {
-        const int rows = opts->segment_rows > 0 ? opts->segment_rows : 4;
+        const int rows = opts->segment_rows > 0 ? opts->segment_rows : 2;
         int v = -1;
         for (int q = 0; q < MIX_NVARIANTS; ++q)
             if (jst::MIX_SEGMENT_ROWS[q] == rows) v = q;

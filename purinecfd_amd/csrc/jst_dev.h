@@ -103,8 +103,8 @@ struct CaseMix {
     const float *vol, *inv_vol, *inv_wd2;
     const double *tnx64, *tny64, *tmx64, *tmy64, *cx64, *cy64; // far-field support (FP64)
     float *ratio[2], *localdt[2]; // frozen per step; [rot] is read by the step, [rot^1] written
-    double *rpart;                // residual partial sums per (4-row block, 58-column group)
-    double *dpart;                // next-dt minima per (4-row block, 58-column group)
+    double *rpart;                // residual partial sums per (2-row block, 58-column group)
+    double *dpart;                // next-dt minima per (2-row block, 58-column group)
     double *res_hist;
     int res_cap;
     CaseCtl *ctl;

@@ -338,7 +338,7 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned q, unsign
 // Every face is evaluated once; the right n face of a lane is its neighbour's left face.
 // MIX_INNER (stages 1-4): U^(k) = U^n - a_k dt/V R in FP32 from the FP32 base dU^n.
 // MIX_LAST (stage 5): FP64 U^{n+1} = U^n - dt/V R, the new FP32 base, the next frozen dt
-// and V/dt, and the step-end partials per (4-row block, 58-column group): summed down each
+// and V/dt, and the step-end partials per (2^bshift-row block, 58-column group): summed down each
 // column in row order, then across the group's columns by a fixed butterfly, so they do not
 // depend on the segment length or the batch.
 // MIX_TERMS: export R = Fc - Fv - Fd and S of the current state (no update).
@@ -346,10 +346,11 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned q, unsign
 template <int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mix_march_kernel(const CaseMix *__restrict__ cases,
                                                         const TileDesc *__restrict__ items, int nitems,
-                                                        int stage, double alpha)
+                                                        int stage, double alpha, int bshift)
 {
     const int lane = threadIdx.x & 63;
     const int nwaves = gridDim.x * 4;
+    const int bmask = (1 << bshift) - 1; // rows per step-end block, minus one
     // the work item is wave-uniform: keep it (and everything derived from it) in SGPRs
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (int it = blockIdx.x * 4 + wave; it < nitems; it += nwaves) {
@@ -415,7 +416,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         float IV[3], VOL[3]; // 1/V and V, rows r-3 .. r-1
         float GTp[5];      // tau face r-3 (previous step)
         float farc[10] = {}; // far-field ghost state (dU, dp, du, dv, dT, nu~)
-        double racc = 0.0;       // stage 5: sum of (rho^{n+1} - rho^n)^2 over this 4-row block
+        double racc = 0.0;       // stage 5: sum of (rho^{n+1} - rho^n)^2 over this row block
         float dacc = __builtin_huge_valf(); // stage 5: minimum next local dt over the block
         // bundle of row r, loaded during step r-1
         float bu[5] = {}, brt = 0.f, btn[2] = {}, bnn[2] = {}, biv = 0.f, bvol = 0.f, biw = 0.f;
@@ -661,7 +662,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     const float acou = cs * (fsqrt(aa * aa + bb * bb) + fsqrt(cc * cc + dd * dd));
                     const float ldt = k.cfl * VOL[0] / (conv + acou);
                     const double dr = u1[0] - former[0];
-                    // this column's residual sum and dt minimum over a 4-row block
+                    // this column's residual sum and dt minimum over a 2^bshift-row block
                     racc += own ? dr * dr : 0.0;
                     dacc = own ? fminf(dacc, ldt) : dacc;
                     if (own) {
@@ -676,7 +677,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
                                                          (unsigned long long)((long long)c * nj + jg));
                     }
-                    if ((c & 3) == 3 || c == ie - 1) { // block end (segments start at multiples of 4)
+                    if ((~c & bmask) == 0 || c == ie - 1) { // block end (segments start at block starts)
                         // sum and minimum over the wave's columns: a fixed butterfly (every lane
                         // ends with the same value), then one partial per (block, column group)
                         double rs = racc;
@@ -687,7 +688,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                             dm = fminf(dm, __shfl_xor(dm, off, 64));
                         }
                         if (lane == 0) {
-                            const unsigned ch = (unsigned)((c >> 2) * ngroups + J0 / MIX_COLS);
+                            const unsigned ch = (unsigned)((c >> bshift) * ngroups + J0 / MIX_COLS);
                             stg(C.rpart, ch, rs);
                             stg(C.dpart, ch, (double)dm);
                         }
@@ -754,13 +755,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 
 // ---------------------------------------------------------------------------------------
 // End of step, one block per case: residual sqrt(sum (rho^{n+1} - rho^n)^2 / NI / NJ)
-// (jst-rs solver.rs:190-200) and the next global dt from the per-(4-row block, column group)
+// (jst-rs solver.rs:190-200) and the next global dt from the per-(2^bshift-row block, column group)
 // partials in a fixed order, totaltime += dt, step += 1, buffer rotation, and the stop decision
 // (convergence / relative drop / step limit, or the non-physical record).
 // ---------------------------------------------------------------------------------------
 template <int NT>
 __global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__ cases,
-                                                     const int *__restrict__ slots)
+                                                     const int *__restrict__ slots, int bshift)
 {
     static_assert(NT == 256, "fixed reduction tree");
     __shared__ double red_s[NT / 64], red_m[NT / 64];
@@ -768,7 +769,7 @@ __global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__
     CaseCtl *ctl = C.ctl;
     const int tid = threadIdx.x;
     if (*(volatile int *)&ctl->status != JST_RUN_ACTIVE) return;
-    const int n = ((C.ni + 3) >> 2) * ((C.nj + MIX_COLS - 1) / MIX_COLS);
+    const int n = ((C.ni + (1 << bshift) - 1) >> bshift) * ((C.nj + MIX_COLS - 1) / MIX_COLS);
     double s = 0.0, m = __builtin_huge_val();
     // sequential per lane (fixed order); unrolled so that the loads are in flight together
 #pragma unroll 16
@@ -893,9 +894,9 @@ __global__ void shadow_kernel(const CaseMix *__restrict__ cases, int ci)
 
 namespace jst {
 
-const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64, 8, 4};
+const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64, 8, 4, 2};
 
-hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
+hipError_t launch_stage_mixed(int bshift, const CaseMix *cases, const TileDesc *items, int nitems, int grid,
                               int stage, double alpha, bool terms, hipStream_t s)
 {
     const int need = (nitems + 3) / 4; // four waves (work items) per workgroup
@@ -903,19 +904,19 @@ hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int n
     if (grid <= 0) return hipSuccess;
     if (terms)
         hipLaunchKernelGGL((mix_march_kernel<MIX_TERMS>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
-                           stage, alpha);
+                           stage, alpha, bshift);
     else if (stage == 4)
         hipLaunchKernelGGL((mix_march_kernel<MIX_LAST>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
-                           stage, alpha);
+                           stage, alpha, bshift);
     else
         hipLaunchKernelGGL((mix_march_kernel<MIX_INNER>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
-                           stage, alpha);
+                           stage, alpha, bshift);
     return hipGetLastError();
 }
 
-hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s)
+hipError_t launch_end_mixed(int bshift, const CaseMix *cases, const int *slots, int nslots, hipStream_t s)
 {
-    hipLaunchKernelGGL((mix_end_kernel<256>), dim3(nslots), dim3(256), 0, s, cases, slots);
+    hipLaunchKernelGGL((mix_end_kernel<256>), dim3(nslots), dim3(256), 0, s, cases, slots, bshift);
     return hipGetLastError();
 }
 
