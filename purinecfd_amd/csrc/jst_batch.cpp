@@ -1137,9 +1137,11 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
             tuned_for = active;
         }
         // Round length: at most steps_per_round steps, ending at the nearest step cap of a
-        // running case or at the next arrival (from the measured step time), so that neither
-        // a retirement nor an admission waits for the rest of a long round.  Results do not
-        // depend on it: every case advances by whole steps on its own data.
+        // running case or at the next arrival still to come (from the measured step time),
+        // so that neither a retirement nor an admission waits for the rest of a long round.
+        // A request that has arrived but is still queued waits for memory or a slot, which
+        // only a retirement frees: it does not shorten the round.  Results do not depend on
+        // the round length: every case advances by whole steps on its own data.
         int nsteps = spr;
         for (int id = 0; id < opts->max_cases; ++id)
             if (slot_req[id] >= 0) {
@@ -1147,9 +1149,11 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
                 if (left > 0 && left < nsteps) nsteps = (int)left;
             }
         if (next < n && step_s > 0.0) {
-            const double wait = std::max(reqs[order[next]].arrival_s - now(), 0.0);
-            const double k = std::ceil(wait / step_s);
-            if (k < (double)nsteps) nsteps = std::max(1, (int)k);
+            const double wait = reqs[order[next]].arrival_s - now();
+            if (wait > 0.0) {
+                const double k = std::ceil(wait / step_s);
+                if (k < (double)nsteps) nsteps = std::max(1, (int)k);
+            }
         }
         const double t_round = now();
         if ((st = jst_batch_step(b, nsteps, 1)) != JST_OK) return st;
