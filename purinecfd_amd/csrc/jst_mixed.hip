@@ -421,6 +421,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         // bundle of row r, loaded during step r-1
         float bu[5] = {}, brt = 0.f, btn[2] = {}, bnn[2] = {}, biv = 0.f, bvol = 0.f, biw = 0.f;
         float bsb[5] = {}, bldt = 0.f;
+        double bfm[5] = {}; // stage 5: U^n of the row the step updates
 
         // Loads of row r's bundle.  In a steady step every condition below holds except that
         // row r may be the first far-field ghost row: its (unused) state load is clamped.
@@ -458,6 +459,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 #pragma unroll
                     for (int m = 0; m < 5; ++m) bsb[m] = bld(r_sb, q, m * pb4);
                 }
+                if (MODE == MIX_LAST) {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) bfm[m] = bld(r_fm, q, m * pb8, 0.0);
+                }
                 if (k.lts) bldt = bld(r_g, q, o_ldt);
             }
         };
@@ -476,14 +481,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             for (int m = 0; m < 5; ++m) {
                 un[m] = bu[m];
                 base[m] = bsb[m];
-            }
-            // stage 5: U^n of cell row r-3, loaded within the step (keeps 10 registers out of
-            // the prefetch bundle) and before the prefetch, so that waiting for it (the
-            // memory counter retires in order) does not wait for the next row's bundle
-            if (MODE == MIX_LAST && (ST || (r - 3 >= is && r - 3 < ie))) {
-                const unsigned q = cidx(r - 3, jw, pitch);
-#pragma unroll
-                for (int m = 0; m < 5; ++m) former[m] = bld(r_fm, q, m * pb8, 0.0);
+                former[m] = bfm[m];
             }
             if (ST || r < ie + 2) load_bundle(r + 1, steady);
             // keep the prefetch and U^n loads ahead of the row's arithmetic (the scheduler
