@@ -209,7 +209,7 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out);
  * 4096 steps. */
 jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, double *res);
 /* Work decomposition of the stage kernel: variant = row-segment length index (16, 24, 32, 40,
- * 48, 64, 8, 4 or 2 rows per wave work item of 58 columns), grid = workgroups of 4 waves (0 = one
+ * 48, 64, 8, 4, 2 or 1 rows per wave work item of 58 columns), grid = workgroups of 4 waves (0 = one
  * per 4 work items).  The grid never changes results; the segment length changes them only at
  * FP32 rounding level (where a segment's edge rows are evaluated). */
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);
@@ -250,7 +250,7 @@ typedef struct {
     int max_cases;            /* case-table slots (concurrent cases) */
     int steps_per_round;      /* steps launched between admission/retirement checks */
     int tune;                 /* retune the launch grid whenever the batch changes size class */
-    int segment_rows;         /* row-segment length of the stage kernel (0: 2, the shortest;
+    int segment_rows;         /* row-segment length of the stage kernel (0: 1, the shortest;
                                  short segments give small cases more waves, hence lower
                                  latency, at some throughput cost on large batches) */
 } jst_serve_opts;

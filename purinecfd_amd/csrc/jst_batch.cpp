@@ -107,7 +107,7 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
-    const size_t nparts = (size_t)((ni + 1) / 2) * ((nj + MIX_COLS - 1) / MIX_COLS); // 2-row blocks at most
+    const size_t nparts = (size_t)ni * ((nj + MIX_COLS - 1) / MIX_COLS); // 1-row blocks at most
     L.rpart = carve(nparts * 8);
     L.dpart = carve(nparts * 8);
     L.res_hist = carve((size_t)res_cap * 8);
@@ -245,9 +245,14 @@ jst_status drop_graph(jst_batch *b)
     return JST_OK;
 }
 
-// log2 of the rows per step-end partial: 4-row blocks, 2-row blocks under 2-row segments (a
-// segment covers whole blocks, so that each partial has one writer)
-int block_shift(const jst_batch *b) { return jst::MIX_SEGMENT_ROWS[b->variant] % 4 == 0 ? 2 : 1; }
+// log2 of the rows per step-end partial: 4-row blocks, 2-row blocks under 2-row segments and
+// 1-row blocks under 1-row segments (a segment covers whole blocks, so that each partial has
+// one writer)
+int block_shift(const jst_batch *b)
+{
+    const int rows = jst::MIX_SEGMENT_ROWS[b->variant];
+    return rows % 4 == 0 ? 2 : (rows % 2 == 0 ? 1 : 0);
+}
 
 // Work items of one case for the row march: 58-column groups x row segments of `rows` rows.
 void case_items(const HostCase &hc, int rows, std::vector<TileDesc> &out)
@@ -1077,7 +1082,7 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     if (st != JST_OK) return st;
     std::unique_ptr<jst_batch> guard(b);
     {
-        const int rows = opts->segment_rows > 0 ? opts->segment_rows : 2;
+        const int rows = opts->segment_rows > 0 ? opts->segment_rows : 1;
         int v = -1;
         for (int q = 0; q < MIX_NVARIANTS; ++q)
             if (jst::MIX_SEGMENT_ROWS[q] == rows) v = q;

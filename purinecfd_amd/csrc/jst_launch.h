@@ -29,11 +29,11 @@ hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, dou
 
 // FP32-mixed row-march variants: rows per work item (one wave per 58 columns x that many
 // rows); the autotuner picks one per batch.  Stage 5 leaves one residual sum and one dt
-// minimum per (block of 2^bshift rows, column group): bshift = 2, or 1 under 2-row segments
-// (a segment covers whole blocks, so that each partial has one writer).  Results
+// minimum per (block of 2^bshift rows, column group): bshift = 2, or 1 under 2-row and 0
+// under 1-row segments (a segment covers whole blocks, so that each partial has one writer).  Results
 // depend on the variant at FP32 rounding level only (where a segment's edge rows run the
 // general row body), never on the launch grid.
-#define MIX_NVARIANTS 9
+#define MIX_NVARIANTS 10
 #define MIX_COLS 58
 extern const int MIX_SEGMENT_ROWS[MIX_NVARIANTS];
 hipError_t launch_stage_mixed(int bshift, const CaseMix *cases, const TileDesc *items, int nitems, int grid,

@@ -892,7 +892,7 @@ __global__ void shadow_kernel(const CaseMix *__restrict__ cases, int ci)
 
 namespace jst {
 
-const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64, 8, 4, 2};
+const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64, 8, 4, 2, 1};
 
 hipError_t launch_stage_mixed(int bshift, const CaseMix *cases, const TileDesc *items, int nitems, int grid,
                               int stage, double alpha, bool terms, hipStream_t s)

@@ -136,7 +136,7 @@ def test_segment_length_and_grid(jst):
     cfg, text = _cases()[1]
     runs = {}
     for variant, grid in ((0, 0), (0, 3), (1, 0), (1, 16), (2, 0), (3, 0), (4, 0), (5, 0), (5, 1024), (6, 0), (6, 5),
-                          (7, 0), (7, 7), (8, 0), (8, 3)):
+                          (7, 0), (7, 7), (8, 0), (8, 3), (9, 0)):
         b = jst.Batch(1 << 30, 2)
         b.set_tiling(variant, grid)
         cid = b.add(cfg, text, 40)
@@ -199,7 +199,7 @@ def test_serving_loop_results_match_alone(jst):
     assert summ["p99_s"] >= summ["p50_s"] > 0
     for (t, cfg, text, steps), r in zip(reqs, res):
         b = jst.Batch(1 << 30, 1)
-        b.set_tiling(8, 0)  # the serving loop's default: 2-row segments
+        b.set_tiling(9, 0)  # the serving loop's default: 1-row segments
         cid = b.add(cfg, text, steps)
         b.run(steps_per_round=10)
         f = b.forces(cid)
