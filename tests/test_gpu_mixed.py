@@ -136,7 +136,7 @@ def test_segment_length_and_grid(jst):
     cfg, text = _cases()[1]
     runs = {}
     for variant, grid in ((0, 0), (0, 3), (1, 0), (1, 16), (2, 0), (3, 0), (4, 0), (5, 0), (5, 1024), (6, 0), (6, 5),
-                          (7, 0), (7, 7), (8, 0), (8, 3), (9, 0)):
+                          (7, 0), (7, 7), (8, 0), (8, 3), (9, 0), (9, 5)):
         b = jst.Batch(1 << 30, 2)
         b.set_tiling(variant, grid)
         cid = b.add(cfg, text, 40)
@@ -220,6 +220,28 @@ def test_admission_by_footprint(jst):
     b.run(steps_per_round=10)
     b.remove(a)
     b.add(cfg, text, 10)
+
+
+def test_serving_requests_wait_for_memory(jst):
+    """An arena that holds one case at a time: requests that have arrived queue until the
+    running case retires, are admitted in arrival order, and still finish with the result
+    they give alone."""
+    cfg, text = _cases()[1]
+    fp = jst.Batch.footprint(cfg, text)
+    reqs = [(0.0, cfg, text, 30), (0.0, cfg, text, 20), (0.01, cfg, text, 25)]
+    res, summ = jst.serve(reqs, hbm_bytes=int(fp * 1.5), max_cases=4, steps_per_round=10)
+    assert summ["completed"] == 3 and summ["max_concurrent"] == 1
+    for q in (1, 2):
+        assert res[q]["admit_s"] >= res[q - 1]["done_s"]
+    for (t, c, m, steps), r in zip(reqs, res):
+        assert r["steps"] == steps and r["status"] == jst.CASE_STOPPED
+    b = jst.Batch(1 << 30, 1)
+    b.set_tiling(9, 0)
+    cid = b.add(cfg, text, 20)
+    b.run(steps_per_round=10)
+    f = b.forces(cid)
+    assert res[1]["cl"] == f["cl"] and res[1]["cd"] == f["cd"]
+    b.close()
 
 
 def test_naca_fine_wall_spacing_tracks_fp64(jst):
