@@ -209,9 +209,9 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out);
  * 4096 steps. */
 jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, double *res);
 /* Work decomposition of the stage kernel: variant = row-segment length index (16, 24, 32, 40,
- * 48, 64, 8, 4, 2 or 1 rows per wave work item of 58 columns), grid = workgroups of 4 waves (0 = one
- * per 4 work items).  The grid never changes results; the segment length changes them only at
- * FP32 rounding level (where a segment's edge rows are evaluated). */
+ * 48, 64, 8, 4, 2 or 1 rows per wave work item of 58 columns), grid = workgroups of 4 waves
+ * (0 = one per 4 work items).  The grid never changes results; the segment length changes
+ * them only at FP32 rounding level (where a segment's edge rows are evaluated). */
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);
 /* Autotuner: times every variant with three grid sizes (0, 512, 1024) on the current batch
  * for trial_steps steps each, in two rounds (each candidate keeps its faster time; the steps
