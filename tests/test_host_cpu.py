@@ -168,6 +168,54 @@ def test_mesh_number_spellings(jst, oracle_lib):
         assert e.value.kind == "BadPoint" and "line 4" in str(e.value), bad
 
 
+def test_mesh_parse_random_texts_match_oracle(jst, oracle_lib):
+    """Seeded random mesh texts, mostly well formed (4 x 8 or 4 x 9 points, the closing
+    duplicate sometimes present), with odd number spellings, CRLF endings, blank lines,
+    extra tokens and the occasional bad token or missing line: the library and the oracle
+    accept the same texts with the same nodes and reject the rest with the same error."""
+    rng = np.random.default_rng(20261203)
+    nums = ["1.5", "+2", "-0.0", "1e400", "-1e-400", "nan", "inf", "3", ".25", "7E+2", "0.1", "-4.", "2e-3"]
+    bad = ["0x1p3", "1e", "abc", "-", "1..2", "--1"]
+    agree_ok = agree_err = 0
+    for _ in range(400):
+        nt = int(rng.integers(8, 10))
+        lines = [("4 %d" % nt) if rng.random() < 0.9 else rng.choice(["4", "x 8", "4 -8", ""])]
+        for r in range(4):
+            first = None
+            for t in range(nt):
+                x, y = rng.choice(nums), rng.choice(nums)
+                if t == 0:
+                    first = (x, y)
+                if t == nt - 1 and nt == 9 and rng.random() < 0.5:
+                    x, y = first  # closing duplicate of the ring's first point
+                if rng.random() < 0.01:
+                    x = rng.choice(bad)
+                line = x + rng.choice([" ", "\t", "  "]) + y + (" z" if rng.random() < 0.05 else "")
+                if rng.random() < 0.995:
+                    lines.append(line + ("\r" if rng.random() < 0.3 else ""))
+                if rng.random() < 0.05:
+                    lines.append("")
+        text = "\n".join(lines) + ("\n" if rng.random() < 0.5 else "")
+        try:
+            r, t, xy = oracle_lib.parse_mesh(text)
+            want = None
+        except oracle_lib.MeshError as e:
+            want = e.code
+        try:
+            m = jst.Mesh.parse(text)
+            got = None
+        except jst.MeshError as e:
+            got = e.status
+        assert got == want, (text, got, want)
+        if want is None:
+            assert (m.n_rings, m.n_theta) == (r, t)
+            assert np.array_equal(m.xy, xy, equal_nan=True)
+            agree_ok += 1
+        else:
+            agree_err += 1
+    assert agree_ok > 100 and agree_err > 20, (agree_ok, agree_err)
+
+
 def test_repository_mesh(jst):
     m = jst.Mesh.parse(mesh_text("fangdata.txt"))
     assert (m.n_rings, m.n_theta, m.n_cells()) == (10, 12, 108)
