@@ -244,6 +244,52 @@ def test_serving_requests_wait_for_memory(jst):
     b.close()
 
 
+def test_tuner_choice_is_valid_and_its_steps_are_real_work(jst):
+    """jst_batch_tune times every (segment length, grid) candidate on the live batch: it
+    returns one of them, leaves every case advanced by the same whole number of steps, and
+    that state matches a fixed-tiling run of as many steps to FP32 rounding."""
+    cases = _cases()
+    b = jst.Batch(1 << 30, 4)
+    ids = [b.add(cfg, text, 10 ** 6) for cfg, text in cases]
+    nvar = 10  # row-segment lengths 16 24 32 40 48 64 8 4 2 1
+    b.set_tiling(nvar - 1, 0)
+    with pytest.raises(jst.JstError):
+        b.set_tiling(nvar, 0)
+    v, g = b.tune(1)
+    assert 0 <= v < nvar and g in (0, 512, 1024)
+    steps = {b.info(cid)["step"] for cid in ids}
+    assert len(steps) == 1
+    n = steps.pop()
+    assert n >= 2 * 3 * nvar  # two rounds x three grids x every variant, >= 1 step each
+    ref = jst.Batch(1 << 30, 4)
+    rids = [ref.add(cfg, text, n) for cfg, text in cases]
+    ref.run(steps_per_round=50)
+    for (cfg, text), a, r in zip(cases, ids, rids):
+        assert b.info(a)["status"] == jst.CASE_RUNNING and ref.info(r)["step"] == n
+        fa, fr = _fields(b, a, text), _fields(ref, r, text)
+        for name in fa:
+            assert rel_l2(fa[name], fr[name]) <= 1e-5, (name, rel_l2(fa[name], fr[name]))
+    b.close()
+    ref.close()
+
+
+def test_serving_with_grid_tuning_matches_alone(jst):
+    """With tuning on, the serving loop re-tunes the launch grid once four or more cases run;
+    the grid never changes results, so every request still ends bitwise where it ends alone."""
+    cases = _cases()
+    reqs = [(0.0, cfg, text, 30 + 5 * q) for q, (cfg, text) in enumerate(cases + cases[:2])]
+    res, summ = jst.serve(reqs, hbm_bytes=1 << 30, max_cases=8, steps_per_round=10, tune=True)
+    assert summ["completed"] == len(reqs) and summ["max_concurrent"] >= 4
+    for (t, cfg, text, steps), r in zip(reqs, res):
+        b = jst.Batch(1 << 30, 1)
+        b.set_tiling(9, 0)
+        cid = b.add(cfg, text, steps)
+        b.run(steps_per_round=10)
+        f = b.forces(cid)
+        assert r["steps"] == steps and r["cl"] == f["cl"] and r["cd"] == f["cd"]
+        b.close()
+
+
 def test_naca_fine_wall_spacing_tracks_fp64(jst):
     """NACA 0012 257x129 with a 1e-5 c first spacing (M 0.75, Re 6e6): the trailing-edge
     wall cells turn nu~ negative in the first steps, which drives the SA r to large negative
