@@ -49,7 +49,7 @@ HBM_PEAK_GBS = 8000.0
 #                  + next dt and V/dt 8 = 160
 BYTES = {"fp64": 184.0, "mixed": 152.0}
 # row-segment length of each stage-kernel tiling variant (jst_batch_set_tiling)
-SEGMENT_ROWS = [16, 24, 32, 40, 48, 64, 8, 4, 2, 1]
+SEGMENT_ROWS = [16, 24, 32, 40, 48, 64, 8, 4, 2, 1, "per-case"]
 DESIGN_BYTES = {"mixed_inner": 92.0, "mixed_last": 160.0}
 METRIC = "cell-stage updates/s FP32-batched vs FP64-plain (ΔCL,ΔCD≤1e-4); p99 solve latency"
 

@@ -208,8 +208,10 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out);
 /* Residual history of a case: res[k] for steps first..first+n-1 (1-based), within the last
  * 4096 steps. */
 jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, double *res);
-/* Work decomposition of the stage kernel: variant = row-segment length index (16, 24, 32, 40,
- * 48, 64, 8, 4, 2 or 1 rows per wave work item of 58 columns), grid = workgroups of 4 waves
+/* Work decomposition of the stage kernel: variant 0..9 = row-segment length (16, 24, 32, 40,
+ * 48, 64, 8, 4, 2 or 1 rows per wave work item of 58 columns) for every case, variant 10 =
+ * each case its own length from its size alone (the shortest of 1, 2, 4 rows whose doubled
+ * length still leaves fewer work items than 1024, else 8); grid = workgroups of 4 waves
  * (0 = one per 4 work items).  The grid never changes results; the segment length changes
  * them only at FP32 rounding level (where a segment's edge rows are evaluated). */
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);
@@ -250,9 +252,10 @@ typedef struct {
     int max_cases;            /* case-table slots (concurrent cases) */
     int steps_per_round;      /* steps launched between admission/retirement checks */
     int tune;                 /* retune the launch grid whenever the batch changes size class */
-    int segment_rows;         /* row-segment length of the stage kernel (0: 1, the shortest;
-                                 short segments give small cases more waves, hence lower
-                                 latency, at some throughput cost on large batches) */
+    int segment_rows;         /* row-segment length of the stage kernel: 0 = each case its
+                                 own, from its size alone (variant 10 of
+                                 jst_batch_set_tiling: short segments for small cases, for
+                                 latency); 1, 2, 4, 8, 16, ... = that length for every case */
 } jst_serve_opts;
 
 typedef struct {

@@ -248,10 +248,27 @@ jst_status drop_graph(jst_batch *b)
 // log2 of the rows per step-end partial: 4-row blocks, 2-row blocks under 2-row segments and
 // 1-row blocks under 1-row segments (a segment covers whole blocks, so that each partial has
 // one writer)
-int block_shift(const jst_batch *b)
+int block_shift_for(int rows) { return rows % 4 == 0 ? 2 : (rows % 2 == 0 ? 1 : 0); }
+
+// A case's own row-segment length, from its size alone: the shortest of 1, 2 and 4 rows whose
+// doubled length would still leave fewer work items than the GPU has SIMDs (1024), else 8
+// rows.  Short segments give a small case the most waves (lowest latency while the GPU is
+// not full); longer ones re-read fewer halo rows.
+int own_segment_rows(int ni, int nj)
 {
-    const int rows = jst::MIX_SEGMENT_ROWS[b->variant];
-    return rows % 4 == 0 ? 2 : (rows % 2 == 0 ? 1 : 0);
+    const long groups = (nj + MIX_COLS - 1) / MIX_COLS;
+    for (int rows : {1, 2, 4})
+        if (groups * ((ni + 2 * rows - 1) / (2 * rows)) < 1024) return rows;
+    return 8;
+}
+
+// Row-segment length of a case under the batch's tiling: the variant's length, or under
+// MIX_VARIANT_PER_CASE the case's own, which depends on nothing else in the batch.
+int case_rows(const jst_batch *b, const HostCase &hc);
+
+int case_rows(const jst_batch *b, const HostCase &hc)
+{
+    return b->variant < MIX_NVARIANTS ? jst::MIX_SEGMENT_ROWS[b->variant] : own_segment_rows(hc.ni, hc.nj);
 }
 
 // Work items of one case for the row march: 58-column groups x row segments of `rows` rows.
@@ -267,13 +284,20 @@ void case_items(const HostCase &hc, int rows, std::vector<TileDesc> &out)
 // row segments, which share halo rows - are dealt to one XCD.
 jst_status rebuild_lists(jst_batch *b)
 {
-    const int rows = jst::MIX_SEGMENT_ROWS[b->variant];
     std::vector<TileDesc> nat;
     std::vector<int> slots;
     for (auto &hc : b->cases) {
         if (!hc) continue;
+        const int rows = case_rows(b, *hc);
         case_items(*hc, rows, nat);
         slots.push_back(hc->slot);
+        // the step-end block size follows the segment length (read by stage 5 and the step end)
+        const int bs = block_shift_for(rows);
+        if (hc->dev.bshift != bs) {
+            hc->dev.bshift = bs;
+            HIPCHK(hipMemcpyAsync(&b->d_cases[hc->slot].bshift, &hc->dev.bshift, sizeof(int),
+                                  hipMemcpyHostToDevice, b->stream));
+        }
     }
     const int nu = ((int)nat.size() + 3) / 4, per = (nu + 7) / 8; // units of 4 items
     std::vector<TileDesc> tiles;
@@ -308,9 +332,9 @@ jst_status rebuild_lists(jst_batch *b)
 jst_status launch_one_step(jst_batch *b)
 {
     for (int k = 0; k < 5; ++k)
-        HIPCHK(jst::launch_stage_mixed(block_shift(b), b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k], false,
+        HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k], false,
                                        b->stream));
-    HIPCHK(jst::launch_end_mixed(block_shift(b), b->d_cases, b->d_slots, b->nslots, b->stream));
+    HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
     return JST_OK;
 }
 
@@ -638,6 +662,7 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     cm.nj = hc->nj;
     cm.pitch = hc->pitch;
     cm.npost = hc->npost;
+    cm.bshift = block_shift_for(case_rows(b, *hc));
     cm.sb = F(L.sb);
     for (int q = 0; q < 2; ++q) {
         cm.u64[q] = D(L.u64[q]);
@@ -810,12 +835,12 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out)
     HostCase *hc = get_case(b, case_id);
     if (!hc) return fail(JST_ERR_ARG, "no such case");
     std::vector<TileDesc> t;
-    case_items(*hc, jst::MIX_SEGMENT_ROWS[b->variant], t);
+    case_items(*hc, case_rows(b, *hc), t);
     TileDesc *dt = nullptr;
     HIPCHK(hipMalloc(&dt, t.size() * sizeof(TileDesc)));
     std::unique_ptr<TileDesc, void (*)(TileDesc *)> guard(dt, [](TileDesc *p) { (void)hipFree(p); });
     HIPCHK(hipMemcpyAsync(dt, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice, b->stream));
-    HIPCHK(jst::launch_stage_mixed(block_shift(b), b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true, b->stream));
+    HIPCHK(jst::launch_stage_mixed(b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true, b->stream));
     jst_status st = fetch_ctl(b); // synchronises the stream before the list is freed
     if (st != JST_OK) return st;
     if (dt_out) *dt_out = b->h_ctl[case_id].dt;
@@ -894,7 +919,7 @@ jst_status jst_batch_get_field(jst_batch *b, int case_id, const char *name, doub
 
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid)
 {
-    if (!b || variant < 0 || variant >= MIX_NVARIANTS || grid < 0) return fail(JST_ERR_ARG, "bad tiling");
+    if (!b || variant < 0 || variant > MIX_VARIANT_PER_CASE || grid < 0) return fail(JST_ERR_ARG, "bad tiling");
     HIPCHK(hipStreamSynchronize(b->stream));
     if (variant != b->variant) b->dirty = true;
     if (grid != b->grid) {
@@ -932,11 +957,11 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
         for (int s = 0; s < nsteps; ++s) {
             for (int k = 0; k < 5; ++k) {
                 HIPCHK(hipEventRecord(evs[10 * s + 2 * k], b->stream));
-                HIPCHK(jst::launch_stage_mixed(block_shift(b), b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
+                HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
                                                false, b->stream));
                 HIPCHK(hipEventRecord(evs[10 * s + 2 * k + 1], b->stream));
             }
-            HIPCHK(jst::launch_end_mixed(block_shift(b), b->d_cases, b->d_slots, b->nslots, b->stream));
+            HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
         }
         HIPCHK(hipStreamSynchronize(b->stream));
         double sum = 0.0;
@@ -978,12 +1003,12 @@ jst_status jst_batch_time_kernels(jst_batch *b, int nsteps, double *inner_ms, do
     for (int s = 0; s < nsteps; ++s) {
         for (int k = 0; k < 5; ++k) {
             HIPCHK(hipEventRecord(evs[12 * s + 2 * k], b->stream));
-            HIPCHK(jst::launch_stage_mixed(block_shift(b), b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
+            HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
                                            false, b->stream));
             HIPCHK(hipEventRecord(evs[12 * s + 2 * k + 1], b->stream));
         }
         HIPCHK(hipEventRecord(evs[12 * s + 10], b->stream));
-        HIPCHK(jst::launch_end_mixed(block_shift(b), b->d_cases, b->d_slots, b->nslots, b->stream));
+        HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
         HIPCHK(hipEventRecord(evs[12 * s + 11], b->stream));
     }
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -1034,7 +1059,7 @@ static jst_status tune_impl(jst_batch *b, int trial_steps, bool variants, int *v
 {
     if (!b || trial_steps <= 0) return fail(JST_ERR_ARG, "bad arguments");
     int best_v = b->variant, best_g = b->grid;
-    const int v0 = variants ? 0 : b->variant, v1 = variants ? MIX_NVARIANTS : b->variant + 1;
+    const int v0 = variants ? 0 : b->variant, v1 = variants ? MIX_VARIANT_PER_CASE + 1 : b->variant + 1;
     // grid 0: one workgroup per four work items; otherwise a persistent grid of k workgroups
     // per CU.  Each candidate is timed in two rounds and keeps its faster time, so that a
     // one-off stall in one timing does not decide the choice.
@@ -1082,10 +1107,10 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     if (st != JST_OK) return st;
     std::unique_ptr<jst_batch> guard(b);
     {
-        const int rows = opts->segment_rows > 0 ? opts->segment_rows : 1;
-        int v = -1;
+        // 0: every case its own segment length (from its size alone), else one fixed length
+        int v = opts->segment_rows == 0 ? MIX_VARIANT_PER_CASE : -1;
         for (int q = 0; q < MIX_NVARIANTS; ++q)
-            if (jst::MIX_SEGMENT_ROWS[q] == rows) v = q;
+            if (jst::MIX_SEGMENT_ROWS[q] == opts->segment_rows) v = q;
         if (v < 0) return fail(JST_ERR_ARG, "unsupported segment_rows");
         if ((st = jst_batch_set_tiling(b, v, 0)) != JST_OK) return st;
     }

@@ -95,6 +95,7 @@ struct MixConst {
 struct CaseMix {
     int ni, nj, pitch;
     int npost;                    // dt-refresh blocks of this case (fixed 1024-cell chunks)
+    int bshift;                   // log2 of the rows per step-end block (from the segment length)
     double *u64[2];               // U^n / U^{n+1}, 5 planes each
     float *sb;                    // FP32 base dU^n = U^n - U_inf, 5 planes
     float *s32[2];                // FP32 stage states dU^(k), 5 planes each (ping-pong)

@@ -840,18 +840,11 @@ jst_status jst_create(const char *config_json, const char *mesh_text, jst_precis
         jst_status sm = jst_case_footprint(config_json, mesh_text, prec, &bytes);
         if (sm != JST_OK) return sm;
         if ((sm = jst_batch_create(JST_MIXED, bytes + (4u << 20), 1, &x->mix)) != JST_OK) return sm;
-        // one case alone: short row segments give it the most waves, longer ones less halo
-        // work per row.  The shortest length whose doubled length would still leave fewer
-        // work items than the GPU has SIMDs (1024): 1 row (variant 9) while 2-row items are
-        // fewer, 2 rows (variant 8) while 4-row items are, 4 rows (variant 7) while 8-row
-        // items are, else 8 rows (variant 6).  Measured: 256x128 cells 67 / 77 / 91 / 120 us
-        // per step with 1 / 2 / 4 / 8 rows; 1024x512 cells 272 / 194 / 173 / 163 us.
-        const int cni = x->mesh.n_rings - 1, cnj = x->mesh.n_theta;
-        const long groups = (cnj + MIX_COLS - 1) / MIX_COLS;
-        const long items2 = groups * ((cni + 1) / 2), items4 = groups * ((cni + 3) / 4),
-                   items8 = groups * ((cni + 7) / 8);
-        const int variant = items2 < 1024 ? 9 : (items4 < 1024 ? 8 : (items8 < 1024 ? 7 : 6));
-        if ((sm = jst_batch_set_tiling(x->mix, variant, 0)) != JST_OK) return sm;
+        // one case alone takes its own row-segment length, chosen from its size: the
+        // shortest of 1, 2, 4 rows whose doubled length still leaves fewer work items than the
+        // GPU has SIMDs, else 8.  Measured: 256x128 cells 67 / 77 / 91 / 120 us per step with
+        // 1 / 2 / 4 / 8 rows; 1024x512 cells 272 / 194 / 173 / 163 us.
+        if ((sm = jst_batch_set_tiling(x->mix, MIX_VARIANT_PER_CASE, 0)) != JST_OK) return sm;
         if ((sm = jst_batch_add(x->mix, config_json, mesh_text, 0, &x->mix_id)) != JST_OK) return sm;
         x->d = finish(x->cfg);
         x->ni = x->mesh.n_rings - 1;

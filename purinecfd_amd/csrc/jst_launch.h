@@ -28,17 +28,20 @@ hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, dou
                               hipStream_t s);
 
 // FP32-mixed row-march variants: rows per work item (one wave per 58 columns x that many
-// rows); the autotuner picks one per batch.  Stage 5 leaves one residual sum and one dt
-// minimum per (block of 2^bshift rows, column group): bshift = 2, or 1 under 2-row and 0
-// under 1-row segments (a segment covers whole blocks, so that each partial has one writer).  Results
+// rows), or MIX_VARIANT_PER_CASE: each case its own length from its size alone; the
+// autotuner picks one per batch.  Stage 5 leaves one residual sum and one dt minimum per
+// (block of 2^bshift rows, column group), bshift per case (CaseMix::bshift): 2, or 1 under
+// 2-row and 0 under 1-row segments (a segment covers whole blocks, so that each partial has
+// one writer).  Results
 // depend on the variant at FP32 rounding level only (where a segment's edge rows run the
 // general row body), never on the launch grid.
 #define MIX_NVARIANTS 10
+#define MIX_VARIANT_PER_CASE MIX_NVARIANTS
 #define MIX_COLS 58
 extern const int MIX_SEGMENT_ROWS[MIX_NVARIANTS];
-hipError_t launch_stage_mixed(int bshift, const CaseMix *cases, const TileDesc *items, int nitems, int grid,
+hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
                               int stage, double alpha, bool terms, hipStream_t s);
-hipError_t launch_end_mixed(int bshift, const CaseMix *cases, const int *slots, int nslots, hipStream_t s);
+hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s);
 hipError_t launch_dt_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, hipStream_t s);
 hipError_t launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);
 hipError_t launch_shadow(const CaseMix *cases, int ci, size_t n5, hipStream_t s);

@@ -346,11 +346,10 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned q, unsign
 template <int MODE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mix_march_kernel(const CaseMix *__restrict__ cases,
                                                         const TileDesc *__restrict__ items, int nitems,
-                                                        int stage, double alpha, int bshift)
+                                                        int stage, double alpha)
 {
     const int lane = threadIdx.x & 63;
     const int nwaves = gridDim.x * 4;
-    const int bmask = (1 << bshift) - 1; // rows per step-end block, minus one
     // the work item is wave-uniform: keep it (and everything derived from it) in SGPRs
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     for (int it = blockIdx.x * 4 + wave; it < nitems; it += nwaves) {
@@ -362,6 +361,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         const unsigned plane = (unsigned)(ni * pitch);
         const int is = w.i0, J0 = w.j0, ie = w.t;
         const int ngroups = (nj + MIX_COLS - 1) / MIX_COLS;
+        const int bshift = C.bshift, bmask = (1 << bshift) - 1; // step-end block: 2^bshift rows
         // control words: read once per item, made wave-uniform so that every pointer and
         // branch derived from them lives in scalar registers
         const int status = __builtin_amdgcn_readfirstlane(*(volatile int *)&ctl->status);
@@ -759,7 +759,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 // ---------------------------------------------------------------------------------------
 template <int NT>
 __global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__ cases,
-                                                     const int *__restrict__ slots, int bshift)
+                                                     const int *__restrict__ slots)
 {
     static_assert(NT == 256, "fixed reduction tree");
     __shared__ double red_s[NT / 64], red_m[NT / 64];
@@ -767,6 +767,7 @@ __global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__
     CaseCtl *ctl = C.ctl;
     const int tid = threadIdx.x;
     if (*(volatile int *)&ctl->status != JST_RUN_ACTIVE) return;
+    const int bshift = C.bshift;
     const int n = ((C.ni + (1 << bshift) - 1) >> bshift) * ((C.nj + MIX_COLS - 1) / MIX_COLS);
     double s = 0.0, m = __builtin_huge_val();
     // sequential per lane (fixed order); unrolled so that the loads are in flight together
@@ -894,7 +895,7 @@ namespace jst {
 
 const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64, 8, 4, 2, 1};
 
-hipError_t launch_stage_mixed(int bshift, const CaseMix *cases, const TileDesc *items, int nitems, int grid,
+hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
                               int stage, double alpha, bool terms, hipStream_t s)
 {
     const int need = (nitems + 3) / 4; // four waves (work items) per workgroup
@@ -902,19 +903,19 @@ hipError_t launch_stage_mixed(int bshift, const CaseMix *cases, const TileDesc *
     if (grid <= 0) return hipSuccess;
     if (terms)
         hipLaunchKernelGGL((mix_march_kernel<MIX_TERMS>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
-                           stage, alpha, bshift);
+                           stage, alpha);
     else if (stage == 4)
         hipLaunchKernelGGL((mix_march_kernel<MIX_LAST>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
-                           stage, alpha, bshift);
+                           stage, alpha);
     else
         hipLaunchKernelGGL((mix_march_kernel<MIX_INNER>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
-                           stage, alpha, bshift);
+                           stage, alpha);
     return hipGetLastError();
 }
 
-hipError_t launch_end_mixed(int bshift, const CaseMix *cases, const int *slots, int nslots, hipStream_t s)
+hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s)
 {
-    hipLaunchKernelGGL((mix_end_kernel<256>), dim3(nslots), dim3(256), 0, s, cases, slots, bshift);
+    hipLaunchKernelGGL((mix_end_kernel<256>), dim3(nslots), dim3(256), 0, s, cases, slots);
     return hipGetLastError();
 }
 

@@ -136,7 +136,7 @@ def test_segment_length_and_grid(jst):
     cfg, text = _cases()[1]
     runs = {}
     for variant, grid in ((0, 0), (0, 3), (1, 0), (1, 16), (2, 0), (3, 0), (4, 0), (5, 0), (5, 1024), (6, 0), (6, 5),
-                          (7, 0), (7, 7), (8, 0), (8, 3), (9, 0), (9, 5)):
+                          (7, 0), (7, 7), (8, 0), (8, 3), (9, 0), (9, 5), (10, 0), (10, 3)):
         b = jst.Batch(1 << 30, 2)
         b.set_tiling(variant, grid)
         cid = b.add(cfg, text, 40)
@@ -199,7 +199,7 @@ def test_serving_loop_results_match_alone(jst):
     assert summ["p99_s"] >= summ["p50_s"] > 0
     for (t, cfg, text, steps), r in zip(reqs, res):
         b = jst.Batch(1 << 30, 1)
-        b.set_tiling(9, 0)  # the serving loop's default: 1-row segments
+        b.set_tiling(10, 0)  # the serving loop's default: each case its own segment length
         cid = b.add(cfg, text, steps)
         b.run(steps_per_round=10)
         f = b.forces(cid)
@@ -236,7 +236,7 @@ def test_serving_requests_wait_for_memory(jst):
     for (t, c, m, steps), r in zip(reqs, res):
         assert r["steps"] == steps and r["status"] == jst.CASE_STOPPED
     b = jst.Batch(1 << 30, 1)
-    b.set_tiling(9, 0)
+    b.set_tiling(10, 0)
     cid = b.add(cfg, text, 20)
     b.run(steps_per_round=10)
     f = b.forces(cid)
@@ -251,7 +251,7 @@ def test_tuner_choice_is_valid_and_its_steps_are_real_work(jst):
     cases = _cases()
     b = jst.Batch(1 << 30, 4)
     ids = [b.add(cfg, text, 10 ** 6) for cfg, text in cases]
-    nvar = 10  # row-segment lengths 16 24 32 40 48 64 8 4 2 1
+    nvar = 11  # row-segment lengths 16 24 32 40 48 64 8 4 2 1, and each case its own
     b.set_tiling(nvar - 1, 0)
     with pytest.raises(jst.JstError):
         b.set_tiling(nvar, 0)
@@ -282,7 +282,7 @@ def test_serving_with_grid_tuning_matches_alone(jst):
     assert summ["completed"] == len(reqs) and summ["max_concurrent"] >= 4
     for (t, cfg, text, steps), r in zip(reqs, res):
         b = jst.Batch(1 << 30, 1)
-        b.set_tiling(9, 0)
+        b.set_tiling(10, 0)
         cid = b.add(cfg, text, steps)
         b.run(steps_per_round=10)
         f = b.forces(cid)
