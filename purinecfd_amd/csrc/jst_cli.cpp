@@ -1,7 +1,8 @@
 // jst-gpu — command-line driver with the flags and outputs of jst-rs's CLI
 // (jst-rs/src/bin/jst.rs:10-123): --mesh --config --steps --out --reslog --quiet --threads
-// --no-output, plus --precision fp64|mixed, --wall PATH (wall Cp/Cf table) and --forces
-// (CL and CD of the final state).  stdout lines, res.log and result.csv use the
+// --no-output, plus --precision fp64|mixed, --wall PATH (wall Cp/Cf table), --forces
+// (CL and CD of the final state) and --profile N (N more steps after the outputs are
+// written, timed with HIP events: ms per step and the mean stage-kernel launch).  stdout lines, res.log and result.csv use the
 // same text formats (Rust `{:6}`, `{:.6e}`, `{:.8e}`), so scripts that read the CPU
 // solver's output read this one's.  Built on the public C ABI only (include/jst_gpu.h).
 #include <algorithm>
@@ -36,6 +37,7 @@ struct Cli {
     long steps = 0;
     bool quiet = false, no_output = false, forces = false;
     long threads = 0;
+    long profile = 0; // steps timed after the run (0: none)
     jst_precision prec = JST_FP64;
 };
 
@@ -45,7 +47,7 @@ int usage(const char *msg)
     fprintf(stderr,
             "Usage: jst-gpu [--mesh PATH] [--config PATH] [--steps N] [--out PATH] [--reslog PATH]\n"
             "               [--quiet] [--threads N] [--no-output] [--precision fp64|mixed]\n"
-            "               [--wall PATH] [--forces]\n");
+            "               [--wall PATH] [--forces] [--profile N]\n");
     return 2;
 }
 
@@ -85,6 +87,10 @@ int main(int argc, char **argv)
         else if (s == "--no-output") cli.no_output = true;
         else if (s == "--wall") cli.wall = val("--wall");
         else if (s == "--forces") cli.forces = true;
+        else if (s == "--profile") {
+            cli.profile = atol(val("--profile"));
+            if (cli.profile <= 0 || cli.profile > 1000000) return usage("--profile takes a step count (1..1000000)");
+        }
         else if (s == "--precision") {
             std::string p = val("--precision");
             if (p == "fp64") cli.prec = JST_FP64;
@@ -169,6 +175,18 @@ int main(int argc, char **argv)
             return 1;
         }
         printf("wall data is written in %s\n", cli.wall.c_str());
+    }
+    if (cli.profile > 0) {
+        // after every output is written: the timed steps advance the state further
+        double total_ms = 0.0, stage_ms = 0.0;
+        if (jst_time_steps(ctx, (int)cli.profile, &total_ms, &stage_ms) != JST_OK) {
+            fprintf(stderr, "Error: %s\n", jst_last_error());
+            jst_destroy(ctx);
+            return 1;
+        }
+        const double ms = total_ms / (double)cli.profile;
+        printf("profile: %ld steps (HIP events): %.4f ms/step, stage kernel %.2f us, %.3e cell-stage updates/s\n",
+               cli.profile, ms, stage_ms * 1e3, 5.0 * (double)cells / (ms * 1e-3));
     }
     jst_destroy(ctx);
     return 0;

@@ -114,3 +114,24 @@ def test_cli_reports_nonphysical(tmp_path):
     assert rc == 1
     assert re.match(r"Error: NonPhysical \{ i: \d+, j: \d+, rho: ", err), err
     assert not (tmp_path / "result.csv").exists()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_cli_profile_times_extra_steps_after_the_outputs(tmp_path, precision):
+    """--profile N times N further steps with HIP events once the outputs are written, so
+    result.csv and res.log are those of the run without it."""
+    common = ["--mesh", str(GOLDEN / "cyl_17x40.txt"), "--config", str(GOLDEN / "config.json"),
+              "--steps", "20", "--quiet", "--precision", precision]
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    rc, out_a, err = run_cli(tmp_path / "a", *common)
+    assert rc == 0, err
+    rc, out_b, err = run_cli(tmp_path / "b", *common, "--profile", "10")
+    assert rc == 0, err
+    m = re.fullmatch(r"profile: 10 steps \(HIP events\): (\d+\.\d{4}) ms/step, stage kernel "
+                     r"(\d+\.\d{2}) us, (\d\.\d{3}e[+-]\d+) cell-stage updates/s", out_b.splitlines()[-1])
+    assert m, out_b.splitlines()[-1]
+    ms, stage_us, rate = float(m.group(1)), float(m.group(2)), float(m.group(3))
+    assert 0.0 < stage_us * 1e-3 * 5 <= 2.0 * ms and rate > 0.0
+    for name in ("result.csv", "res.log"):
+        assert (tmp_path / "a" / name).read_text() == (tmp_path / "b" / name).read_text(), name
