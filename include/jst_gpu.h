@@ -198,6 +198,14 @@ jst_status jst_batch_remove(jst_batch *b, int case_id);
 jst_status jst_batch_step(jst_batch *b, int nsteps, int sync);
 jst_status jst_batch_info(const jst_batch *b, int case_id, jst_case_info *out);
 jst_status jst_batch_forces(jst_batch *b, int case_id, double *cl, double *cd);
+/* State injection into a resident case (no reference counterpart; the batch form of
+ * jst_set_state, and its FP32-mixed implementation): u_planes = 5 x NI x NJ conserved
+ * variables, row-major per component, become U^n; the FP32 base and the frozen time step
+ * are re-derived from it.  The Solver::unpack check then runs (solver.rs:151-171): a
+ * non-physical cell fills err and returns JST_ERR_NONPHYSICAL.  With jst_batch_get_field's
+ * U0..U4 this is checkpoint / resume of a case. */
+jst_status jst_batch_set_state(jst_batch *b, int case_id, const double *u_planes,
+                               jst_nonphysical *err);
 /* jst_wall for one case of a batch (FP64 wall terms from the case's FP64 state). */
 jst_status jst_batch_wall(jst_batch *b, int case_id, double *x, double *y, double *cp, double *cf);
 /* Field export for one case (names as jst_get_field: state, geometry, and after

@@ -817,6 +817,34 @@ jst_status jst_batch_wall(jst_batch *b, int case_id, double *x, double *y, doubl
     return batch_wall(b, case_id, x, y, cp, cf);
 }
 
+jst_status jst_batch_set_state(jst_batch *b, int case_id, const double *u_planes,
+                               jst_nonphysical *err)
+{
+    if (!b || !u_planes) return fail(JST_ERR_ARG, "NULL argument");
+    HostCase *hc = get_case(b, case_id);
+    if (!hc) return fail(JST_ERR_ARG, "no such case");
+    jst_status st = fetch_ctl(b);
+    if (st != JST_OK) return st;
+    // U^n into the current FP64 state, then the FP32 base and the frozen dt / V/dt of it
+    const int rot = b->h_ctl[hc->slot].rot;
+    const size_t ncell = (size_t)hc->ni * hc->nj;
+    std::vector<double> h(5 * hc->plane, 0.0);
+    for (int m = 0; m < 5; ++m)
+        for (int i = 0; i < hc->ni; ++i)
+            for (int j = 0; j < hc->nj; ++j)
+                h[m * hc->plane + (size_t)i * hc->pitch + j] = u_planes[m * ncell + (size_t)i * hc->nj + j];
+    HIPCHK(hipMemcpyAsync(hc->dev.u64[rot], h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                          b->stream));
+    if ((st = set_ctl_field(b, hc->slot, &CaseCtl::err_key, (unsigned long long)JST_ERR_NONE)) != JST_OK)
+        return st;
+    if ((st = set_ctl_field(b, hc->slot, &CaseCtl::status, (int)JST_RUN_ACTIVE)) != JST_OK) return st;
+    HIPCHK(jst::launch_shadow(b->d_cases, hc->slot, 5 * hc->plane, b->stream));
+    HIPCHK(jst::launch_dt_mixed(b->d_cases, hc->d_posts, hc->npost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if ((st = check_injected_state(u_planes, hc->ni, hc->nj, hc->cfg.gamma, err)) != JST_OK) return st;
+    return fetch_ctl(b);
+}
+
 jst_status jst_batch_forces(jst_batch *b, int case_id, double *cl, double *cd)
 {
     double f[4];

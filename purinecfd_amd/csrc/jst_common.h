@@ -54,6 +54,12 @@ jst_status wall_distribution(const double *d_terms, const jst::GeomPtrs &g, int 
                              const Config &c, const Derived &d, hipStream_t s, double *x,
                              double *y, double *cp, double *cf);
 
+// Solver::unpack check of an injected state (solver.rs:151-171): the row-major first cell
+// with rho or p NaN or <= 1e-15 fills err and fails with JST_ERR_NONPHYSICAL.  u_planes is
+// 5 x NI x NJ, row-major per component.
+jst_status check_injected_state(const double *u_planes, int ni, int nj, double gamma,
+                                jst_nonphysical *err);
+
 // single-case use of a batch (the FP32-mixed jst_ctx)
 jst_status batch_forces(jst_batch *b, int case_id, double out[4]);
 jst_status batch_wall(jst_batch *b, int case_id, double *x, double *y, double *cp, double *cf);

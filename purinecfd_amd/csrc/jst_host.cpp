@@ -783,6 +783,37 @@ jst_status report_nonphysical(jst_ctx *x, jst_nonphysical *err)
 // ------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------
+namespace jsth {
+
+jst_status check_injected_state(const double *u_planes, int ni, int nj, double gamma,
+                                jst_nonphysical *err)
+{
+    const size_t ncell = (size_t)ni * nj;
+    for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj; ++j) {
+            double u[5];
+            for (int m = 0; m < 5; ++m) u[m] = u_planes[m * ncell + (size_t)i * nj + j];
+            const double rho = u[0];
+            bool bad = std::isnan(rho) || rho <= 1e-15;
+            double p = NAN;
+            if (!bad) {
+                const double inv = 1.0 / rho, vx = u[1] * inv, vy = u[2] * inv;
+                p = (gamma - 1.0) * (u[3] - rho * (vx * vx + vy * vy) * 0.5);
+                bad = std::isnan(p) || p <= 1e-15;
+            }
+            if (bad) {
+                if (err) *err = jst_nonphysical{i, j, rho, p};
+                char buf[256];
+                snprintf(buf, sizeof buf, "non-physical state at cell (%d, %d): rho = %s, p = %s", i,
+                         j, rust_exp(rho, -1).c_str(), rust_exp(p, -1).c_str());
+                return fail(JST_ERR_NONPHYSICAL, buf);
+            }
+        }
+    return JST_OK;
+}
+
+} // namespace jsth
+
 extern "C" {
 
 const char *jst_last_error(void) { return g_err.c_str(); }
@@ -1197,7 +1228,7 @@ jst_status jst_get_field(jst_ctx *x, const char *name, double *dst, size_t n)
 jst_status jst_set_state(jst_ctx *x, const double *u_planes, jst_nonphysical *err)
 {
     if (!x || !u_planes) return fail(JST_ERR_ARG, "NULL argument");
-    if (x->mix) return fail(JST_ERR_ARG, "set_state is available on the FP64 plain path");
+    if (x->mix) return jst_batch_set_state(x->mix, x->mix_id, u_planes, err);
     jst_status st = ctl_fetch(x);
     if (st != JST_OK) return st;
     const int ni = x->ni, nj = x->nj, pitch = x->pitch;
@@ -1211,27 +1242,7 @@ jst_status jst_set_state(jst_ctx *x, const double *u_planes, jst_nonphysical *er
                           hipMemcpyHostToDevice, x->stream));
     if ((st = ctl_set(x, &CaseCtl::err_key, (unsigned long long)JST_ERR_NONE)) != JST_OK) return st;
     if ((st = ctl_set(x, &CaseCtl::status, (int)JST_RUN_ACTIVE)) != JST_OK) return st;
-    // Solver::unpack check, row-major first offender (solver.rs:151-171)
-    for (int i = 0; i < ni; ++i)
-        for (int j = 0; j < nj; ++j) {
-            double u[5];
-            for (int m = 0; m < 5; ++m) u[m] = u_planes[m * ncell + (size_t)i * nj + j];
-            double rho = u[0];
-            bool bad = std::isnan(rho) || rho <= 1e-15;
-            double p = NAN;
-            if (!bad) {
-                double inv = 1.0 / rho, vx = u[1] * inv, vy = u[2] * inv;
-                p = (x->cfg.gamma - 1.0) * (u[3] - rho * (vx * vx + vy * vy) * 0.5);
-                bad = std::isnan(p) || p <= 1e-15;
-            }
-            if (bad) {
-                if (err) *err = jst_nonphysical{i, j, rho, p};
-                char buf[256];
-                snprintf(buf, sizeof buf, "non-physical state at cell (%d, %d): rho = %s, p = %s", i,
-                         j, rust_exp(rho, -1).c_str(), rust_exp(p, -1).c_str());
-                return fail(JST_ERR_NONPHYSICAL, buf);
-            }
-        }
+    if ((st = check_injected_state(u_planes, ni, nj, x->cfg.gamma, err)) != JST_OK) return st;
     if ((st = refresh_dt(x)) != JST_OK) return st;
     return ctl_fetch(x);
 }

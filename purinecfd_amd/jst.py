@@ -33,7 +33,7 @@ EXPORTS = ["jst_last_error", "jst_config_check", "jst_mesh_parse", "jst_create",
            "jst_residual", "jst_residual_terms", "jst_get_field", "jst_set_state", "jst_forces",
            "jst_wall", "jst_write_wall_csv", "jst_write_csv", "jst_progress", "jst_time_steps",
            "jst_batch_create", "jst_batch_destroy", "jst_case_footprint", "jst_batch_add",
-           "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces",
+           "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces", "jst_batch_set_state",
            "jst_batch_wall", "jst_batch_get_field", "jst_batch_residual_terms", "jst_batch_history",
            "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_steps", "jst_batch_time_kernels",
            "jst_batch_counts",
@@ -147,6 +147,7 @@ def lib():
         L.jst_batch_step.argtypes = [vp, ip, ip]
         L.jst_batch_info.argtypes = [vp, ip, C.POINTER(_CaseInfo)]
         L.jst_batch_forces.argtypes = [vp, ip, dp, dp]
+        L.jst_batch_set_state.argtypes = [vp, ip, dp, C.POINTER(_NonPhys)]
         L.jst_batch_wall.argtypes = [vp, ip, dp, dp, dp, dp]
         L.jst_batch_get_field.argtypes = [vp, ip, C.c_char_p, dp, sz]
         L.jst_batch_residual_terms.argtypes = [vp, ip, dp]
@@ -427,6 +428,15 @@ class Batch:
         cl, cd = C.c_double(), C.c_double()
         _check(lib().jst_batch_forces(self._h, cid, C.byref(cl), C.byref(cd)))
         return {"cl": cl.value, "cd": cd.value}
+
+    def set_state(self, cid: int, u_planes: np.ndarray) -> None:
+        """Inject U (5 x NI x NJ) as the case's current state (checkpoint / resume)."""
+        u = np.ascontiguousarray(u_planes, dtype=np.float64)
+        rec = _NonPhys()
+        st = lib().jst_batch_set_state(self._h, cid, _dp(u), C.byref(rec))
+        if st == JST_ERR_NONPHYSICAL:
+            raise NonPhysical(_err(), rec)
+        _check(st)
 
     def wall(self, cid: int, nj: int) -> dict:
         """jst_wall for case cid (nj = its wall face count)."""

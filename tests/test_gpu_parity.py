@@ -250,9 +250,10 @@ def test_blowup_is_reported_not_nan(jst, oracle_lib):
     assert (ei.value.i, ei.value.j) == (eo.value.i, eo.value.j)
 
 
-def test_set_state_reports_nonphysical(jst):
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_set_state_reports_nonphysical(jst, precision):
     """solver.rs:358-365: rho = -1 injected at (0, 0) is reported, not fatal."""
-    s = jst.Solver(config_text(), mesh_text("fangdata.txt"))
+    s = jst.Solver(config_text(), mesh_text("fangdata.txt"), jst.FP64 if precision == "fp64" else jst.MIXED)
     u = np.stack([s.get(f"U{k}") for k in range(5)])
     u[0, 0] = -1.0
     with pytest.raises(jst.NonPhysical) as ei:
@@ -272,3 +273,25 @@ def test_write_csv_format(jst, tmp_path):
     first = lines[1].split(",")
     assert first[:2] == ["1", "1"] and "e" in first[4] and "+" not in first[4]
     assert abs(float(first[4]) - s.get("rho")[0]) <= 1e-8 * s.get("rho")[0]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_resume_from_an_exported_state(jst, precision):
+    """Checkpoint and resume through the C ABI: U exported after 15 steps (U0..U4) and
+    injected into a fresh solver (jst_set_state) continues as the uninterrupted run does.
+    The frozen time step of the next step is recomputed from the injected state; on the
+    FP64 path that reproduces the uninterrupted run to 1e-13 after 15 more steps, on the
+    FP32-mixed path to FP32 rounding (the FP32 base and dt are re-derived from FP64 U)."""
+    prec = jst.FP64 if precision == "fp64" else jst.MIXED
+    text = mesh_text("cyl_17x40.txt")
+    a = jst.Solver(config_text(), text, prec)
+    a.run(15)
+    u = np.stack([a.get(f"U{k}") for k in range(5)])
+    a.run(15)
+    b = jst.Solver(config_text(), text, prec)
+    b.set_state(u)
+    b.run(15)
+    tol = 1e-13 if precision == "fp64" else 1e-6
+    for n in ("rho", "u", "v", "p", "nut"):
+        x, y = b.get(n), a.get(n)
+        assert np.max(np.abs(x - y)) <= tol * np.max(np.abs(y)), (n, np.max(np.abs(x - y)))
