@@ -740,7 +740,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         for (; r < rs0 && r <= ie + 2; ++r) row_step(r, edge_t{});
         // stage 5: two steady rows per trip, so that the row windows rename across the pair
         // instead of moving (820 instead of 889 instructions per row; 1.6 % faster).  Stages
-        // 1-4 also drop from 670 to 607 instructions per row that way, but run 5 % slower.
+        // 1-4 also drop from 670 to 607 instructions per row that way, but run 10 % slower
+        // (105 -> 115 us on the 64-case batch, with or without a scheduling barrier between
+        // the two rows).
         if (MODE == MIX_LAST)
             for (; r + 1 <= rs1; r += 2) {
                 row_step(r, steady_t{});
