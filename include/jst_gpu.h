@@ -260,10 +260,14 @@ typedef struct {
     int max_cases;            /* case-table slots (concurrent cases) */
     int steps_per_round;      /* steps launched between admission/retirement checks */
     int tune;                 /* retune the launch grid whenever the batch changes size class */
-    int segment_rows;         /* row-segment length of the stage kernel: 0 = each case its
-                                 own, from its size alone (variant 10 of
-                                 jst_batch_set_tiling: short segments for small cases, for
-                                 latency); 1, 2, 4, 8, 16, ... = that length for every case */
+    int segment_rows;         /* row-segment length of the stage kernel.  0 = adaptive: each
+                                 case its own length from its size alone (variant 10 of
+                                 jst_batch_set_tiling: short segments, for latency) while
+                                 the GPU has waves to spare, 24 rows for every case while it
+                                 is saturated (fewer halo rows, for throughput); a switch
+                                 moves results at FP32 rounding level.  -1 = each case its
+                                 own length always (results bitwise independent of the
+                                 load).  1, 2, 4, 8, 16, ... = that length for every case */
 } jst_serve_opts;
 
 typedef struct {
@@ -271,6 +275,7 @@ typedef struct {
     double cell_stages;       /* sum over cases of NI*NJ*5*steps */
     double p50_s, p99_s;      /* time to convergence (done - arrival) over completed requests */
     int completed, nonphysical, max_concurrent;
+    int tiling_switches;      /* adaptive segment_rows: times the batch changed tiling */
 } jst_serve_summary;
 
 /* Serving loop: requests are admitted in arrival order as slots and arena memory free up,

@@ -1134,9 +1134,12 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     jst_status st = jst_batch_create(JST_MIXED, opts->hbm_bytes, opts->max_cases, &b);
     if (st != JST_OK) return st;
     std::unique_ptr<jst_batch> guard(b);
+    // 0: every case its own segment length (from its size alone) while the GPU has waves to
+    // spare, 24-row segments for every case while it is saturated; < 0: every case its own
+    // length always (results bitwise independent of the load); else one fixed length
+    const bool adaptive = opts->segment_rows == 0;
     {
-        // 0: every case its own segment length (from its size alone), else one fixed length
-        int v = opts->segment_rows == 0 ? MIX_VARIANT_PER_CASE : -1;
+        int v = opts->segment_rows <= 0 ? MIX_VARIANT_PER_CASE : -1;
         for (int q = 0; q < MIX_NVARIANTS; ++q)
             if (jst::MIX_SEGMENT_ROWS[q] == opts->segment_rows) v = q;
         if (v < 0) return fail(JST_ERR_ARG, "unsupported segment_rows");
@@ -1155,6 +1158,7 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     double cell_stages = 0.0;
     int next = 0, done = 0, nonphys = 0, maxc = 0, active = 0, tuned_for = -1;
     double step_s = 0.0; // measured wall time per step of the current mix (round average)
+    int switches = 0;    // adaptive tiling changes
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
     auto now = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
@@ -1186,6 +1190,32 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
                 if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(std::min(wait, 0.05)));
             }
             continue;
+        }
+        if (adaptive) {
+            // Step time of each tiling, in row steps of one wave: a work item of L rows walks
+            // L + 5 rows, and the launch lasts about the longer of one item and the total
+            // over the 2048 waves two per SIMD keep resident.  Short own lengths win while the
+            // GPU has waves to spare; 24-row segments (fewer halo rows) once it is saturated.
+            // The tiling changes only when the other one is estimated 20 % faster.
+            double lat_total = 0.0, thr_total = 0.0, lat_item = 0.0;
+            for (auto &hc : b->cases)
+                if (hc) {
+                    const long groups = (hc->nj + MIX_COLS - 1) / MIX_COLS;
+                    const int rows = own_segment_rows(hc->ni, hc->nj);
+                    lat_total += (double)groups * (double)((hc->ni + rows - 1) / rows) * (rows + 5);
+                    lat_item = std::max(lat_item, (double)std::min(rows, hc->ni) + 5.0);
+                    thr_total += (double)groups * ((double)hc->ni + 5.0 * ((hc->ni + 23) / 24));
+                }
+            const double t_lat = std::max(lat_item, lat_total / 2048.0);
+            const double t_thr = std::max(29.0, thr_total / 2048.0);
+            const int v24 = 1; // MIX_SEGMENT_ROWS[1] == 24
+            int want = b->variant;
+            if (b->variant == MIX_VARIANT_PER_CASE && t_thr < 0.8 * t_lat) want = v24;
+            if (b->variant == v24 && t_lat < 0.8 * t_thr) want = MIX_VARIANT_PER_CASE;
+            if (want != b->variant) {
+                if ((st = jst_batch_set_tiling(b, want, b->grid)) != JST_OK) return st;
+                ++switches;
+            }
         }
         if (opts->tune && admitted && active != tuned_for && active >= 4) {
             int v, g;
@@ -1264,6 +1294,7 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
         summary->completed = (int)lat.size();
         summary->nonphysical = nonphys;
         summary->max_concurrent = maxc;
+        summary->tiling_switches = switches;
     }
     return JST_OK;
 }

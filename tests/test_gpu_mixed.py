@@ -309,3 +309,28 @@ def test_naca_fine_wall_spacing_tracks_fp64(jst):
     assert np.array_equal(wm["x"], w64["x"]) and np.array_equal(wm["y"], w64["y"])
     assert np.max(np.abs(wm["cp"] - w64["cp"])) <= 1e-4
     assert np.max(np.abs(wm["cf"] - w64["cf"])) <= 1e-4 * max(1.0, np.max(np.abs(w64["cf"])))
+
+
+def test_serving_under_load_switches_tiling_within_tolerance(jst):
+    """Thirty requests arriving at once saturate the GPU at the cases' own (short) segment
+    lengths, so the adaptive serving loop (segment_rows 0, the default) runs them with
+    24-row segments; each result then differs from the same case alone at most at FP32
+    rounding (|dCL|, |dCD| <= 1e-5 here), inside the |dCL|, |dCD| <= 1e-4 contract."""
+    cfg_b = _cases()[1][0]
+    text = ellipse_mesh(129, 256, 1.04)  # 128 x 256 cells: 640 one-row work items each
+    reqs = [(0.0, cfg_b, text, 20)] * 30
+    res, summ = jst.serve(reqs, hbm_bytes=4 << 30, max_cases=32, steps_per_round=10)
+    assert summ["completed"] == 30 and summ["nonphysical"] == 0
+    assert summ["tiling_switches"] >= 1
+    b = jst.Batch(1 << 30, 1)
+    b.set_tiling(10, 0)
+    cid = b.add(cfg_b, text, 20)
+    b.run(steps_per_round=10)
+    f = b.forces(cid)
+    b.close()
+    for r in res:
+        assert r["steps"] == 20
+        assert abs(r["cl"] - f["cl"]) <= 1e-5 and abs(r["cd"] - f["cd"]) <= 1e-5
+    # a light stream keeps each case's own segment length: no switch
+    res, summ = jst.serve(reqs[:2], hbm_bytes=4 << 30, max_cases=32, steps_per_round=10)
+    assert summ["tiling_switches"] == 0
