@@ -250,7 +250,7 @@ def main() -> int:
     ap.add_argument("--mixed", action="store_true", help="single-workload mode: FP32-mixed path")
     ap.add_argument("--acc-steps", type=int, default=100)
     ap.add_argument("--plain-steps", type=int, default=20)
-    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--throughput-only", action="store_true",
