@@ -18,7 +18,7 @@ extern "C" {
 #endif
 
 /* Status codes.  1..6 map 1:1 onto jst-rs MeshError (mesh.rs:18-25); CONFIG is the
- * InvalidData io::Error of Config::from_str (config.rs:243-248); NONPHYSICAL is
+ * InvalidData io::Error of Config::from_str (config.rs:165-170); NONPHYSICAL is
  * NonPhysical (state.rs:241-259, solver.rs:151-185). */
 typedef enum {
     JST_OK = 0,
@@ -63,7 +63,7 @@ typedef void (*jst_step_cb)(long step, double residual, void *user);
  * (mesh.rs:27-44, state.rs:249-257).  Never NULL. */
 const char *jst_last_error(void);
 
-/* Config::from_str (config.rs:243-248) + Config::finish (251-284) on the host only.
+/* Config::from_str (config.rs:165-170) + Config::finish (173-206) on the host only.
  * derived_out (may be NULL) receives 20 doubles: cv cp cw1 cv1_cubed c_inf u_inf v_inf
  * rho_inf mu_inf nut_inf supersonic suth_num inv_t_ref inv_pr inv_prt inv_kappa2 cw3_6
  * one_plus_cw3_6 mu0_eff halo.  No GPU is touched. */
@@ -79,12 +79,12 @@ jst_status jst_mesh_parse(const char *mesh_text, int *n_rings, int *n_theta, dou
  * Mesh::parse and returns their status. */
 jst_status jst_create(const char *config_json, const char *mesh_text, jst_precision prec,
                       jst_ctx **out);
-/* Solver::from_paths (solver.rs:289-293). */
+/* Solver::from_paths (solver.rs:64-69). */
 jst_status jst_create_from_paths(const char *config_path, const char *mesh_path,
                                  jst_precision prec, jst_ctx **out);
 void jst_destroy(jst_ctx *ctx);
 
-/* Solver::{ni, nj} (solver.rs:295-306). */
+/* Solver::{ni, nj} (solver.rs:71-78). */
 jst_status jst_dims(const jst_ctx *ctx, int *ni, int *nj);
 
 /* nsteps x Solver::advance (solver.rs:97-120) followed by Solver::residual (190-200):
@@ -273,13 +273,21 @@ typedef struct {
 typedef struct {
     double wall_s;
     double cell_stages;       /* sum over cases of NI*NJ*5*steps */
-    double p50_s, p99_s;      /* time to convergence (done - arrival) over completed requests */
+    double p50_s, p99_s;      /* time to completion (done - arrival) over every retired
+                                 request: converged, stopped at its step limit or
+                                 non-physical */
     int completed, nonphysical, max_concurrent;
     int tiling_switches;      /* adaptive segment_rows: times the batch changed tiling */
+    int converged;            /* requests retired as JST_CASE_CONVERGED */
+    int rejected;             /* requests never admitted (invalid, or larger than the arena) */
+    double p50_converged_s, p99_converged_s; /* time to convergence (done - arrival) over the
+                                                converged requests only; NaN if none */
 } jst_serve_summary;
 
 /* Serving loop: requests are admitted in arrival order as slots and arena memory free up,
- * run concurrently in one batch, and retired on convergence / step limit / divergence. */
+ * run concurrently in one batch, and retired on convergence / step limit / divergence.  A
+ * request that is invalid, or that does not fit the arena even when no other case is
+ * resident, is rejected (its result status -1) and the loop goes on with the rest. */
 jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
                      jst_request_result *results, jst_serve_summary *summary);
 

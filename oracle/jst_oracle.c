@@ -27,14 +27,14 @@
 
 void jo_config_defaults_ext(jo_config *c)
 {
-    c->nut_ratio = 0.1; /* config.rs:274 hard-codes 0.1 */
+    c->nut_ratio = 0.1; /* config.rs:196 hard-codes 0.1 */
     c->reynolds = 0.0;
     c->ref_length = 1.0;
     c->local_time_stepping = 0;
     c->residual_drop = 0.0;
 }
 
-/* config.rs:231-234: mu0 * (r*sqrt r) * (T0+Ts) / (T+Ts) */
+/* config.rs:153-156: mu0 * (r*sqrt r) * (T0+Ts) / (T+Ts) */
 double jo_sutherland(double mu0, double t, double t_ref, double t_suth)
 {
     double r = t / t_ref;
@@ -58,7 +58,7 @@ static double powi_c(double x, int n)
     return have ? res : 1.0;
 }
 
-/* config.rs:251-284 (Config::finish).  EXT: Reynolds rescale of mu0 and nut_ratio. */
+/* config.rs:173-206 (Config::finish).  EXT: Reynolds rescale of mu0 and nut_ratio. */
 void jo_config_finish(const jo_config *c, jo_derived *d)
 {
     double cv = c->r_gas / (c->gamma - 1.0);
@@ -97,7 +97,7 @@ void jo_config_finish(const jo_config *c, jo_derived *d)
     d->one_plus_cw3_6 = 1.0 + powi_c(c->cw3, 6);
 }
 
-/* config.rs:288-291 (Config::mu) */
+/* config.rs:210-213 (Config::mu) */
 double jo_mu(const jo_config *c, const jo_derived *d, double t)
 {
     double r = t * d->inv_t_ref;
@@ -1009,9 +1009,35 @@ static int unpack(jo_solver *s, double *err)
     return JO_OK;
 }
 
+/* Test hooks: the two halves of advance() (solver.rs:97-120) so that a single RK stage can
+ * be observed.  jo_begin_step takes the step-start snapshot and the frozen dt; jo_stage runs
+ * one stage (residual terms at the current state, update from the snapshot, unpack). */
+double jo_begin_step(jo_solver *s)
+{
+    memcpy(s->u_former, s->u, sizeof(double) * 5 * s->ncell);
+    memcpy(s->rho_prev, s->rho, sizeof(double) * s->ncell);
+    return jo_timestep(s);
+}
+
+int jo_stage(jo_solver *s, double alpha, double dt, double *err)
+{
+    jo_residual_terms(s);
+    update(s, alpha, dt);
+    return unpack(s, err);
+}
+
+void jo_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int jo_advance(jo_solver *s, double *err)
 {
-    static const double RK_ALPHA[5] = {0.25, 1.0 / 6.0, 0.375, 0.5, 1.0}; /* config.rs:295 */
+    static const double RK_ALPHA[5] = {0.25, 1.0 / 6.0, 0.375, 0.5, 1.0}; /* config.rs:217 */
     int st, k;
     memcpy(s->u_former, s->u, sizeof(double) * 5 * s->ncell);
     memcpy(s->rho_prev, s->rho, sizeof(double) * s->ncell);

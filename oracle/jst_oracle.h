@@ -132,6 +132,12 @@ void jo_forces(jo_solver *s, double *out4);
 void jo_wall(jo_solver *s, double *x, double *y, double *cp, double *cf);
 /* number of OpenMP threads the oracle will use */
 int jo_threads(void);
+/* set the OpenMP thread count (n <= 0: unchanged) */
+void jo_set_threads(int n);
+/* advance() split for tests: step-start snapshot + frozen global dt (returned), then one RK
+ * stage U = U_former - alpha*dt*R/V (alpha*localdt with local time stepping) + unpack */
+double jo_begin_step(jo_solver *s);
+int jo_stage(jo_solver *s, double alpha, double dt, double *err);
 
 #ifdef __cplusplus
 }

@@ -100,6 +100,11 @@ def lib():
         L.jo_mu.argtypes = [C.POINTER(JoConfig), C.POINTER(JoDerived), C.c_double]
         L.jo_mu.restype = C.c_double
         L.jo_threads.restype = C.c_int
+        L.jo_set_threads.argtypes = [C.c_int]
+        L.jo_begin_step.argtypes = [vp]
+        L.jo_begin_step.restype = C.c_double
+        L.jo_stage.argtypes = [vp, C.c_double, C.c_double, dp]
+        L.jo_stage.restype = C.c_int
         _lib = L
     return _lib
 
@@ -207,6 +212,16 @@ class OracleSolver:
 
     def timestep(self) -> float:
         return lib().jo_timestep(self._h)
+
+    def begin_step(self) -> float:
+        """Step-start snapshot and frozen dt (the first half of advance())."""
+        return lib().jo_begin_step(self._h)
+
+    def stage(self, alpha: float, dt: float) -> None:
+        """One RK stage from the snapshot of begin_step()."""
+        err = (C.c_double * 4)()
+        if lib().jo_stage(self._h, alpha, dt, err) != 0:
+            raise NonPhysical(list(err))
 
     def residual_terms(self) -> None:
         lib().jo_residual_terms(self._h)

@@ -35,7 +35,7 @@ using namespace jsth;
 
 namespace {
 
-const double RK_ALPHA[5] = {0.25, 1.0 / 6.0, 0.375, 0.5, 1.0}; // jst-rs config.rs:295
+const double RK_ALPHA[5] = {0.25, 1.0 / 6.0, 0.375, 0.5, 1.0}; // jst-rs config.rs:217
 const int GRAPH_STEPS = 16;
 
 inline size_t r256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -624,7 +624,7 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     if (st != JST_OK) return st;
     st = parse_mesh(mesh_text, mesh);
     if (st != JST_OK) return st;
-    if (hc->cfg.halo != JST_H) return fail(JST_ERR_CONFIG, "unsupported halo: kernels are built for IM=3");
+    if (hc->cfg.halo < JST_H) return fail(JST_ERR_CONFIG, "unsupported halo: the stencils need IM >= 3");
     if (mesh.n_rings - 1 < JST_H || mesh.n_theta < 4)
         return fail(JST_ERR_ARG, "the mixed path needs at least 3 cell rings and 4 cells per ring");
     // 32-bit byte offsets: every array of a case (at most 6 planes of (NI+1) x pitch, 8
@@ -823,8 +823,10 @@ jst_status jst_batch_set_state(jst_batch *b, int case_id, const double *u_planes
     if (!b || !u_planes) return fail(JST_ERR_ARG, "NULL argument");
     HostCase *hc = get_case(b, case_id);
     if (!hc) return fail(JST_ERR_ARG, "no such case");
-    jst_status st = fetch_ctl(b);
+    // the unpack check first: a rejected state leaves the case as it was
+    jst_status st = check_injected_state(u_planes, hc->ni, hc->nj, hc->cfg.gamma, err);
     if (st != JST_OK) return st;
+    if ((st = fetch_ctl(b)) != JST_OK) return st;
     // U^n into the current FP64 state, then the FP32 base and the frozen dt / V/dt of it
     const int rot = b->h_ctl[hc->slot].rot;
     const size_t ncell = (size_t)hc->ni * hc->nj;
@@ -840,8 +842,6 @@ jst_status jst_batch_set_state(jst_batch *b, int case_id, const double *u_planes
     if ((st = set_ctl_field(b, hc->slot, &CaseCtl::status, (int)JST_RUN_ACTIVE)) != JST_OK) return st;
     HIPCHK(jst::launch_shadow(b->d_cases, hc->slot, 5 * hc->plane, b->stream));
     HIPCHK(jst::launch_dt_mixed(b->d_cases, hc->d_posts, hc->npost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    if ((st = check_injected_state(u_planes, hc->ni, hc->nj, hc->cfg.gamma, err)) != JST_OK) return st;
     return fetch_ctl(b);
 }
 
@@ -1169,7 +1169,10 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
             const jst_request &r = reqs[order[next]];
             int id = -1;
             st = jst_batch_add(b, r.config_json, r.mesh_text, r.max_steps, &id);
-            if (st == JST_ERR_NOSPACE) break;
+            // no space while other cases run: wait for a retirement.  No space in an empty
+            // batch (whole arena free, every slot free): the request can never fit, so it is
+            // rejected like an invalid one instead of being retried forever.
+            if (st == JST_ERR_NOSPACE && active > 0) break;
             if (st != JST_OK) {
                 results[order[next]].status = -1;
                 ++done;
@@ -1276,25 +1279,35 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
         if (!progressed) return fail(JST_ERR_HIP, "the device completed no step of an active case");
     }
     if (summary) {
-        std::vector<double> lat;
+        // latency = done - arrival; over every retired request, and over the converged ones
+        std::vector<double> lat, lat_conv;
         for (int q = 0; q < n; ++q)
-            if (results[q].done_s >= 0 && results[q].status >= 0)
+            if (results[q].done_s >= 0 && results[q].status >= 0) {
                 lat.push_back(results[q].done_s - reqs[q].arrival_s);
+                if (results[q].status == JST_CASE_CONVERGED) lat_conv.push_back(lat.back());
+            }
         std::sort(lat.begin(), lat.end());
-        auto pct = [&](double p) -> double {
-            if (lat.empty()) return (double)NAN;
-            const double idx = p * (lat.size() - 1);
+        std::sort(lat_conv.begin(), lat_conv.end());
+        auto pct = [](const std::vector<double> &v, double p) -> double {
+            if (v.empty()) return (double)NAN;
+            const double idx = p * (v.size() - 1);
             const size_t lo = (size_t)std::floor(idx), hi = (size_t)std::ceil(idx);
-            return lat[lo] + (lat[hi] - lat[lo]) * (idx - lo);
+            return v[lo] + (v[hi] - v[lo]) * (idx - lo);
         };
         summary->wall_s = now();
         summary->cell_stages = cell_stages;
-        summary->p50_s = pct(0.50);
-        summary->p99_s = pct(0.99);
+        summary->p50_s = pct(lat, 0.50);
+        summary->p99_s = pct(lat, 0.99);
         summary->completed = (int)lat.size();
         summary->nonphysical = nonphys;
         summary->max_concurrent = maxc;
         summary->tiling_switches = switches;
+        summary->converged = (int)lat_conv.size();
+        summary->p50_converged_s = pct(lat_conv, 0.50);
+        summary->p99_converged_s = pct(lat_conv, 0.99);
+        int rejected = 0;
+        for (int q = 0; q < n; ++q) rejected += results[q].status < 0 ? 1 : 0;
+        summary->rejected = rejected;
     }
     return JST_OK;
 }

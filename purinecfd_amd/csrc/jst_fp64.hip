@@ -175,7 +175,7 @@ __device__ __forceinline__ void diffusion_tensor(const PhysConst &k, double t, d
                                                  double d[8], double &chi_out, double &fv1_out)
 {
     double r = t * k.inv_t_ref;
-    double mu = r * sqrt(r) * k.suth_num / (t + k.t_suth); // config.rs:288-291
+    double mu = r * sqrt(r) * k.suth_num / (t + k.t_suth); // config.rs:210-213
     double chi = rho_nu / mu;
     double c3 = chi * chi * chi;
     double f1 = c3 / (c3 + k.cv1_cubed);

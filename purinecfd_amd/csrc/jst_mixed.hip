@@ -633,7 +633,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     if (own) {
 #pragma unroll
                         for (int m = 0; m < 5; ++m) bst(r_dst, q, m * pb4, nw[m]);
-                        // NonPhysical (state.rs:100-117), also catching NaN
+                        // NonPhysical (solver.rs:151-185, state.rs:241-259), also catching NaN
                         if (!(rho > 1e-15f) || !(prho > 1e-15f * rho))
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
                                                          (unsigned long long)((long long)c * nj + jg));

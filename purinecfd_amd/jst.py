@@ -101,7 +101,9 @@ class _ServeOpts(C.Structure):
 class _ServeSummary(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("cell_stages", C.c_double), ("p50_s", C.c_double),
                 ("p99_s", C.c_double), ("completed", C.c_int), ("nonphysical", C.c_int),
-                ("max_concurrent", C.c_int), ("tiling_switches", C.c_int)]
+                ("max_concurrent", C.c_int), ("tiling_switches", C.c_int),
+                ("converged", C.c_int), ("rejected", C.c_int),
+                ("p50_converged_s", C.c_double), ("p99_converged_s", C.c_double)]
 
 
 STEP_CB = C.CFUNCTYPE(None, C.c_long, C.c_double, C.c_void_p)
@@ -508,7 +510,9 @@ def serve(requests, hbm_bytes: int = 16 << 30, max_cases: int = 64, steps_per_ro
     return out, {"wall_s": summ.wall_s, "cell_stages": summ.cell_stages, "p50_s": summ.p50_s,
                  "p99_s": summ.p99_s, "completed": summ.completed,
                  "nonphysical": summ.nonphysical, "max_concurrent": summ.max_concurrent,
-                 "tiling_switches": summ.tiling_switches}
+                 "tiling_switches": summ.tiling_switches, "converged": summ.converged,
+                 "rejected": summ.rejected, "p50_converged_s": summ.p50_converged_s,
+                 "p99_converged_s": summ.p99_converged_s}
 
 
 def copy_probe(nbytes: int) -> float:

@@ -153,7 +153,7 @@ def test_cylinder_c1_matches_oracle(jst, oracle_lib):
 
 
 def test_run_is_deterministic(jst):
-    """properties.rs:49-75 / solver.rs:343-350: bitwise reproducible runs."""
+    """properties.rs:49-75 / solver.rs:318-326: bitwise reproducible runs."""
     text = ellipse_mesh(17, 48, 1.1)
     a = jst.Solver(config_text(), text)
     b = jst.Solver(config_text(), text)

@@ -1,6 +1,6 @@
 """Host logic of libjstgpu.so on the CPU (no GPU touched): exports, config and mesh parsing.
 
-The config checks mirror jst-rs config.rs:297-356; the mesh checks mirror mesh.rs:192-260
+The config checks mirror jst-rs config.rs:219-278; the mesh checks mirror mesh.rs:192-260
 and are cross-checked against the oracle's mesh parser.
 """
 from __future__ import annotations
@@ -55,7 +55,7 @@ def test_config_derived_matches_oracle_bitwise(jst, oracle_lib):
 
 
 def test_cw1_closure_and_sutherland_reference(jst, oracle_lib):
-    """config.rs:332-355: Cw1 closure; mu(T0) = mu0."""
+    """config.rs:254-278: Cw1 closure; mu(T0) = mu0."""
     doc = config_dict()
     d = jst.Config.from_str(json.dumps(doc)).derived
     sa = doc["spalart_allmaras"]
