@@ -227,6 +227,14 @@ jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);
  * for trial_steps steps each, in two rounds (each candidate keeps its faster time; the steps
  * are real work and advance the cases), and keeps the fastest; returns the choice. */
 jst_status jst_batch_tune(jst_batch *b, int trial_steps, int *variant, int *grid);
+/* Benchmark hooks that let the caller time steps with its own clock: jst_batch_time_begin
+ * switches off the convergence and step-limit stops of every running case, builds the
+ * launch lists and the HIP graphs that queueing nsteps steps uses and launches each graph
+ * once (up to GRAPH_STEPS + 1 untimed warm steps), then waits; jst_batch_step(b, nsteps, 1)
+ * is then exactly nsteps steps of every running case plus one wait; jst_batch_time_end
+ * restores the stops and fails with JST_ERR_NONPHYSICAL if a case diverged meanwhile. */
+jst_status jst_batch_time_begin(jst_batch *b, int nsteps);
+jst_status jst_batch_time_end(jst_batch *b);
 /* Benchmark hook: nsteps steps of every running case with convergence stops disabled;
  * total_ms from HIP events around the region, stage_ms (may be NULL) the mean duration of
  * one stage launch from a second pass with events around every stage launch. */

@@ -212,6 +212,7 @@ struct jst_batch {
     double *d_scratch_out = nullptr;
     CaseDev *d_view = nullptr; // FP64 view of one case for the wall-force kernel
     hipEvent_t ev[2] = {nullptr, nullptr};
+    std::vector<int> timing_slots; // cases whose stops jst_batch_time_begin switched off
 
     ~jst_batch()
     {
@@ -959,19 +960,49 @@ jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid)
     return JST_OK;
 }
 
-jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, double *stage_ms)
+jst_status jst_batch_time_begin(jst_batch *b, int nsteps)
 {
     if (!b || nsteps <= 0) return fail(JST_ERR_ARG, "bad arguments");
     jst_status st = fetch_ctl(b);
     if (st != JST_OK) return st;
-    std::vector<int> touched;
+    b->timing_slots.clear();
     for (auto &hc : b->cases)
-        if (hc && b->h_ctl[hc->slot].status == JST_RUN_ACTIVE) touched.push_back(hc->slot);
-    if ((st = set_ctl_fields(b, touched, &CaseCtl::timing, 1)) != JST_OK) return st;
+        if (hc && b->h_ctl[hc->slot].status == JST_RUN_ACTIVE) b->timing_slots.push_back(hc->slot);
+    if ((st = set_ctl_fields(b, b->timing_slots, &CaseCtl::timing, 1)) != JST_OK) return st;
     if (b->dirty && (st = rebuild_lists(b)) != JST_OK) return st;
-    // build (and warm) the graphs outside the timed region
+    // build the graphs that queueing nsteps steps uses and launch each once, so that neither
+    // capture, instantiation nor a graph's first upload falls inside the timed steps
     if ((st = prepare_graphs(b, nsteps)) != JST_OK) return st;
-    if ((st = launch_steps(b, 1)) != JST_OK) return st;
+    if (nsteps >= GRAPH_STEPS && (st = launch_steps(b, GRAPH_STEPS)) != JST_OK) return st;
+    if ((nsteps % GRAPH_STEPS || nsteps < GRAPH_STEPS) && (st = launch_steps(b, 1)) != JST_OK) return st;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return JST_OK;
+}
+
+jst_status jst_batch_time_end(jst_batch *b)
+{
+    if (!b) return fail(JST_ERR_ARG, "bad arguments");
+    const std::vector<int> touched = b->timing_slots;
+    b->timing_slots.clear();
+    jst_status st = set_ctl_fields(b, touched, &CaseCtl::timing, 0);
+    if (st != JST_OK) return st;
+    if ((st = fetch_ctl(b)) != JST_OK) return st;
+    // a diverged case stops its own kernels early: a timing that includes one is not valid
+    for (int slot : touched)
+        if (b->h_ctl[slot].err_key != JST_ERR_NONE) {
+            const long long cell = (long long)(b->h_ctl[slot].err_key & 0xFFFFFFFFull);
+            char buf[160];
+            snprintf(buf, sizeof buf, "non-physical state in case %d at cell (%lld, %lld) during timing",
+                     slot, cell / b->cases[slot]->nj, cell % b->cases[slot]->nj);
+            return fail(JST_ERR_NONPHYSICAL, buf);
+        }
+    return JST_OK;
+}
+
+jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, double *stage_ms)
+{
+    jst_status st = jst_batch_time_begin(b, nsteps);
+    if (st != JST_OK) return st;
     HIPCHK(hipEventRecord(b->ev[0], b->stream));
     if ((st = launch_steps(b, nsteps)) != JST_OK) return st;
     HIPCHK(hipEventRecord(b->ev[1], b->stream));
@@ -1001,18 +1032,7 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
         for (auto &e : evs) (void)hipEventDestroy(e);
         *stage_ms = sum / (5.0 * nsteps);
     }
-    if ((st = set_ctl_fields(b, touched, &CaseCtl::timing, 0)) != JST_OK) return st;
-    if ((st = fetch_ctl(b)) != JST_OK) return st;
-    // a diverged case stops its own kernels early: a timing that includes one is not valid
-    for (int slot : touched)
-        if (b->h_ctl[slot].err_key != JST_ERR_NONE) {
-            const long long cell = (long long)(b->h_ctl[slot].err_key & 0xFFFFFFFFull);
-            char buf[160];
-            snprintf(buf, sizeof buf, "non-physical state in case %d at cell (%lld, %lld) during timing",
-                     slot, cell / b->cases[slot]->nj, cell % b->cases[slot]->nj);
-            return fail(JST_ERR_NONPHYSICAL, buf);
-        }
-    return JST_OK;
+    return jst_batch_time_end(b);
 }
 
 jst_status jst_batch_time_kernels(jst_batch *b, int nsteps, double *inner_ms, double *last_ms,

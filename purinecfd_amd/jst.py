@@ -35,7 +35,8 @@ EXPORTS = ["jst_last_error", "jst_config_check", "jst_mesh_parse", "jst_create",
            "jst_batch_create", "jst_batch_destroy", "jst_case_footprint", "jst_batch_add",
            "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces", "jst_batch_set_state",
            "jst_batch_wall", "jst_batch_get_field", "jst_batch_residual_terms", "jst_batch_history",
-           "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_steps", "jst_batch_time_kernels",
+           "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_begin", "jst_batch_time_end",
+           "jst_batch_time_steps", "jst_batch_time_kernels",
            "jst_batch_counts",
            "jst_copy_probe", "jst_serve"]
 JST_ERR_NOSPACE = 11
@@ -156,6 +157,8 @@ def lib():
         L.jst_batch_history.argtypes = [vp, ip, C.c_long, C.c_long, dp]
         L.jst_batch_set_tiling.argtypes = [vp, ip, ip]
         L.jst_batch_tune.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
+        L.jst_batch_time_begin.argtypes = [vp, ip]
+        L.jst_batch_time_end.argtypes = [vp]
         L.jst_batch_time_steps.argtypes = [vp, ip, dp, dp]
         L.jst_batch_time_kernels.argtypes = [vp, ip, dp, dp, dp]
         L.jst_batch_counts.argtypes = [vp, C.POINTER(ip), C.POINTER(C.c_long)]
@@ -475,6 +478,13 @@ class Batch:
         _check(lib().jst_batch_time_steps(self._h, int(nsteps), C.byref(total),
                                           C.byref(stage) if per_stage else None))
         return total.value, (stage.value if per_stage else None)
+
+    def time_begin(self, nsteps: int) -> None:
+        """Untimed set-up of nsteps timed steps (stops off, graphs built and warmed)."""
+        _check(lib().jst_batch_time_begin(self._h, int(nsteps)))
+
+    def time_end(self) -> None:
+        _check(lib().jst_batch_time_end(self._h))
 
     def time_kernels(self, nsteps: int):
         """Mean launch durations (ms) of RK stages 1-4, stage 5 and the step end."""

@@ -1,0 +1,179 @@
+"""The benchmark configurations on the GPU, each against the CPU oracle.
+
+* NACA 0012 257 x 129 (M 0.75, AoA 2 deg, Re 6e6, first spacing 1e-5 c): FP64 plain path
+  against the oracle over 40 steps, per residual evaluation at 1e-10.
+* NACA 0012 1025 x 513 (M 0.8, AoA 1.5 deg, Re 1e7, first spacing 2e-6 c): FP64 plain at
+  1e-10 and FP32-mixed at FP32 level against the oracle.
+* The 64-case uneven batch (NACA 4-digit 129 x 65 .. 513 x 257, seed 20261203): every case on
+  the FP64 plain path against the oracle after 20 steps (1e-10), and all 64 in one FP32-mixed
+  batched launch sequence against the oracle after 20 steps and against the FP64 plain path
+  after 100 steps, |dCL|, |dCD| <= 1e-4 and field L2 per case.
+* The serving stream (Poisson arrivals, mean 20 requests/s, 2 s, seed 20261204): each result
+  bitwise the same as the case run alone, converged cylinders within 1e-4 of the oracle.
+
+The model for the checks is jst-rs tests/golden.rs:221-275 (terms of one residual evaluation,
+field-normed) and tests/properties.rs:49-75 (bitwise independence of the partition).
+"""
+from __future__ import annotations
+
+import json
+
+import numpy as np
+import pytest
+
+from conftest import field_cmp
+
+pytestmark = pytest.mark.gpu
+
+TERMS = [f"{t}{k}" for t in ("Fc", "Fv", "Fd") for k in range(5)] + ["S"]
+
+
+@pytest.fixture(scope="module")
+def jst():
+    from purinecfd_amd import jst as mod
+    mod.lib()
+    return mod
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _fp64_vs_oracle(jst, oracle_lib, w, steps, label):
+    s = jst.Solver(w["config"], w["mesh"], jst.FP64)
+    o = oracle_lib.OracleSolver(json.loads(w["config"]), w["mesh"])
+    hist = []
+    rep = s.run(steps, on_step=lambda k, r: hist.append(r))
+    orep = o.run(steps)
+    assert rep.steps == orep["steps"] == steps, label
+    field_cmp(np.array(hist), orep["history"], f"{label} residual history", 1e-9)
+    assert abs(rep.totaltime - o.totaltime) <= 1e-12 * o.totaltime
+    dt = s.residual_terms()
+    odt = o.timestep()
+    o.residual_terms()
+    assert abs(dt - odt) <= 1e-12 * odt, label
+    for t in TERMS:
+        field_cmp(s.get(t), o.get(t), f"{label} {t}", 1e-10, 1e-280)
+    for name in ("rho", "p", "u", "v", "nut"):
+        field_cmp(s.get(name), o.get(name), f"{label} {name}", 1e-10, 1e-300)
+    f, of = s.forces(), o.forces()
+    assert abs(f["cl"] - of["cl"]) <= 1e-8 and abs(f["cd"] - of["cd"]) <= 1e-8, (label, f, of)
+    return s, o
+
+
+def test_naca0012_257x129_fp64_matches_oracle(jst, oracle_lib):
+    from purinecfd_amd import workloads
+    w = workloads.c2_naca0012()
+    _fp64_vs_oracle(jst, oracle_lib, w, 40, "naca0012_257x129")
+
+
+def test_naca0012_1025x513_both_paths_match_oracle(jst, oracle_lib):
+    """The roofline case: FP64 plain at 1e-10 after 10 steps; FP32-mixed after the same 10
+    steps within FP32 level of the oracle (residual R = Fc - Fv - Fd field-normed <= 2e-4,
+    state L2 <= 1e-5, |dCL|, |dCD| <= 1e-5)."""
+    from purinecfd_amd import workloads
+    w = workloads.c3_naca_large()
+    s64, o = _fp64_vs_oracle(jst, oracle_lib, w, 10, "naca0012_1025x513")
+    s64.close()
+    smx = jst.Solver(w["config"], w["mesh"], jst.MIXED)
+    assert smx.run(10).steps == 10
+    for name in ("rho", "p", "u", "v"):
+        assert rel_l2(smx.get(name), o.get(name)) <= 1e-5, name
+    f, of = smx.forces(), o.forces()
+    assert abs(f["cl"] - of["cl"]) <= 1e-5 and abs(f["cd"] - of["cd"]) <= 1e-5, (f, of)
+    smx.residual_terms()  # on the mixed state; the oracle's terms are at its own state
+    for k in range(5):
+        want = o.get(f"Fc{k}") - o.get(f"Fv{k}") - o.get(f"Fd{k}")
+        err = float(np.max(np.abs(smx.get(f"R{k}") - want)) / np.max(np.abs(want)))
+        assert err <= 2e-4, (k, err)
+
+
+def test_uneven_batch_of_64_cases(jst, oracle_lib):
+    """All 64 cases: FP64 plain vs oracle after 20 steps (residual history 1e-9, state and
+    wall forces 1e-10 / 1e-8); the FP32-mixed batch (all 64 cases in every launch, 24-row
+    segments as the benchmark runs them) vs the oracle after 20 steps and vs the FP64 plain
+    path after 100 steps: |dCL|, |dCD| <= 1e-4, field L2 <= 1e-5 per case."""
+    from purinecfd_amd import workloads
+    cases = workloads.c4_batch()
+    assert len(cases) == 64
+    b = jst.Batch(hbm_bytes=8 << 30, max_cases=64)
+    b.set_tiling(1, 512)  # 24-row segments
+    ids = [b.add(w["config"], w["mesh"], 100) for w in cases]
+    b.step(20)
+    worst = {"fp64_vs_oracle": 0.0, "mixed_vs_oracle": 0.0, "dcl": 0.0, "dcd": 0.0, "l2": 0.0}
+    plain = []
+    for w, cid in zip(cases, ids):
+        assert b.info(cid)["step"] == 20, w["name"]
+        o = oracle_lib.OracleSolver(json.loads(w["config"]), w["mesh"])
+        orep = o.run(20)
+        assert orep["steps"] == 20, w["name"]
+        s = jst.Solver(w["config"], w["mesh"], jst.FP64)
+        hist = []
+        assert s.run(20, on_step=lambda k, r: hist.append(r)).steps == 20
+        field_cmp(np.array(hist), orep["history"], f"{w['name']} residual history", 1e-9)
+        for name in ("rho", "p", "u", "v", "nut"):
+            want = o.get(name)
+            got = s.get(name)
+            field_cmp(got, want, f"{w['name']} fp64 {name}", 1e-10, 1e-300)
+            worst["fp64_vs_oracle"] = max(worst["fp64_vs_oracle"],
+                                          float(np.max(np.abs(got - want)) / np.max(np.abs(want))))
+            if name != "nut":
+                e = rel_l2(b.get(cid, name, w["ni"], w["nj"]), want)
+                worst["mixed_vs_oracle"] = max(worst["mixed_vs_oracle"], e)
+                assert e <= 1e-5, (w["name"], name, e)
+        f, of, fm = s.forces(), o.forces(), b.forces(cid)
+        assert abs(f["cl"] - of["cl"]) <= 1e-8 and abs(f["cd"] - of["cd"]) <= 1e-8, w["name"]
+        assert abs(fm["cl"] - of["cl"]) <= 1e-4 and abs(fm["cd"] - of["cd"]) <= 1e-4, w["name"]
+        del o
+        plain.append(s)
+    b.step(80)
+    for w, cid, s in zip(cases, ids, plain):
+        assert b.info(cid)["step"] == 100
+        assert s.run(80).steps == 80
+        f, fm = s.forces(), b.forces(cid)
+        dcl, dcd = abs(fm["cl"] - f["cl"]), abs(fm["cd"] - f["cd"])
+        worst["dcl"] = max(worst["dcl"], dcl)
+        worst["dcd"] = max(worst["dcd"], dcd)
+        assert dcl <= 1e-4 and dcd <= 1e-4, (w["name"], dcl, dcd)
+        for name in ("rho", "p", "u", "v"):
+            e = rel_l2(b.get(cid, name, w["ni"], w["nj"]), s.get(name))
+            worst["l2"] = max(worst["l2"], e)
+            assert e <= 1e-5, (w["name"], name, e)
+        s.close()
+    b.close()
+    print("64-case batch:", {k: f"{v:.2e}" for k, v in worst.items()})
+
+
+def test_serving_stream_matches_alone_runs_and_the_oracle(jst, oracle_lib):
+    """A seeded 2 s Poisson stream of the serving mix (cylinders 40x17 .. 160x65 to a
+    5-decade drop, draws of the uneven NACA distribution capped at 800 steps), served with
+    each case's own segment length (segment_rows -1): every request ends bitwise where the same
+    case run alone ends (steps, status, residual, CL, CD); one converged cylinder of each size
+    is within 1e-4 of the oracle in CL and CD."""
+    from purinecfd_amd import workloads
+    reqs = workloads.c5_requests(seconds=2.0, rate=20.0, seed=20261204)
+    assert len(reqs) >= 20
+    res, summ = jst.serve([(r["arrival_s"], r["config"], r["mesh"], r["max_steps"]) for r in reqs],
+                          hbm_bytes=16 << 30, max_cases=96, steps_per_round=50, segment_rows=-1)
+    assert summ["completed"] == len(reqs) and summ["rejected"] == 0 and summ["nonphysical"] == 0
+    assert summ["converged"] >= 1 and summ["p99_converged_s"] >= summ["p50_converged_s"] > 0
+    checked = set()
+    for r, out in zip(reqs, res):
+        b = jst.Batch(1 << 30, 1)
+        b.set_tiling(10, 0)  # each case its own segment length, as in the stream
+        cid = b.add(r["config"], r["mesh"], r["max_steps"])
+        b.run(steps_per_round=50)
+        info, f = b.info(cid), b.forces(cid)
+        b.close()
+        assert out["steps"] == info["step"] and out["status"] == info["status"], r["name"]
+        assert out["residual"] == info["residual"], r["name"]
+        assert out["cl"] == f["cl"] and out["cd"] == f["cd"], r["name"]
+        if r["name"].startswith("cyl") and out["status"] == jst.CASE_CONVERGED \
+                and r["name"] not in checked:
+            checked.add(r["name"])
+            o = oracle_lib.OracleSolver(json.loads(r["config"]), r["mesh"])
+            orep = o.run(r["max_steps"])
+            assert orep["converged"] and abs(orep["steps"] - out["steps"]) <= 2, r["name"]
+            of = o.forces()
+            assert abs(out["cl"] - of["cl"]) <= 1e-4 and abs(out["cd"] - of["cd"]) <= 1e-4, r["name"]
+    assert checked, "no cylinder converged"

@@ -194,9 +194,11 @@ def test_serving_loop_results_match_alone(jst):
     cases = _cases()
     reqs = [(0.0, cases[0][0], cases[0][1], 80), (0.05, cases[1][0], cases[1][1], 60),
             (0.1, cases[2][0], cases[2][1], 70), (0.12, cases[0][0], cases[0][1], 40)]
-    res, summ = jst.serve(reqs, hbm_bytes=1 << 30, max_cases=3, steps_per_round=10)
-    assert summ["completed"] == 4 and summ["nonphysical"] == 0
+    res, summ = jst.serve(reqs, hbm_bytes=1 << 30, max_cases=3, steps_per_round=10,
+                          segment_rows=-1)  # own lengths at any load: bitwise vs alone
+    assert summ["completed"] == 4 and summ["nonphysical"] == 0 and summ["rejected"] == 0
     assert summ["p99_s"] >= summ["p50_s"] > 0
+    assert summ["converged"] == 0 and np.isnan(summ["p50_converged_s"])  # all step-capped
     for (t, cfg, text, steps), r in zip(reqs, res):
         b = jst.Batch(1 << 30, 1)
         b.set_tiling(10, 0)  # the serving loop's default: each case its own segment length
@@ -229,7 +231,8 @@ def test_serving_requests_wait_for_memory(jst):
     cfg, text = _cases()[1]
     fp = jst.Batch.footprint(cfg, text)
     reqs = [(0.0, cfg, text, 30), (0.0, cfg, text, 20), (0.01, cfg, text, 25)]
-    res, summ = jst.serve(reqs, hbm_bytes=int(fp * 1.5), max_cases=4, steps_per_round=10)
+    res, summ = jst.serve(reqs, hbm_bytes=int(fp * 1.5), max_cases=4, steps_per_round=10,
+                          segment_rows=-1)
     assert summ["completed"] == 3 and summ["max_concurrent"] == 1
     for q in (1, 2):
         assert res[q]["admit_s"] >= res[q - 1]["done_s"]
@@ -278,7 +281,8 @@ def test_serving_with_grid_tuning_matches_alone(jst):
     the grid never changes results, so every request still ends bitwise where it ends alone."""
     cases = _cases()
     reqs = [(0.0, cfg, text, 30 + 5 * q) for q, (cfg, text) in enumerate(cases + cases[:2])]
-    res, summ = jst.serve(reqs, hbm_bytes=1 << 30, max_cases=8, steps_per_round=10, tune=True)
+    res, summ = jst.serve(reqs, hbm_bytes=1 << 30, max_cases=8, steps_per_round=10, tune=True,
+                          segment_rows=-1)
     assert summ["completed"] == len(reqs) and summ["max_concurrent"] >= 4
     for (t, cfg, text, steps), r in zip(reqs, res):
         b = jst.Batch(1 << 30, 1)
@@ -334,3 +338,20 @@ def test_serving_under_load_switches_tiling_within_tolerance(jst):
     # a light stream keeps each case's own segment length: no switch
     res, summ = jst.serve(reqs[:2], hbm_bytes=4 << 30, max_cases=32, steps_per_round=10)
     assert summ["tiling_switches"] == 0
+
+
+def test_serving_rejects_a_request_larger_than_the_arena(jst):
+    """A request whose footprint exceeds the whole arena can never be admitted: it is
+    rejected (status -1) once nothing else is resident, and the other requests, before and
+    after it, still run to the result they give alone."""
+    cfg, text = _cases()[1]
+    big = ellipse_mesh(257, 512, 1.02)
+    fp = jst.Batch.footprint(cfg, text)
+    assert jst.Batch.footprint(cfg, big) > 2 * fp
+    reqs = [(0.0, cfg, text, 20), (0.0, cfg, big, 20), (0.01, cfg, text, 25)]
+    res, summ = jst.serve(reqs, hbm_bytes=int(fp * 1.5), max_cases=4, steps_per_round=10,
+                          segment_rows=-1)
+    assert res[1]["status"] == -1 and summ["rejected"] == 1
+    assert summ["completed"] == 2
+    for q, steps in ((0, 20), (2, 25)):
+        assert res[q]["status"] == jst.CASE_STOPPED and res[q]["steps"] == steps

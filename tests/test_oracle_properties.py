@@ -21,13 +21,13 @@ plausible mistake breaks:
 """
 from __future__ import annotations
 
-import copy
 import math
 
 import numpy as np
 import pytest
 
-from conftest import config_dict, ellipse_mesh
+from conftest import (check_mass_conservation, config_dict, ellipse_mesh, ring_mass_flux,
+                      rotate_mesh)
 
 
 def _solver(oracle_lib, rings, nj, stretch, **over):
@@ -35,32 +35,6 @@ def _solver(oracle_lib, rings, nj, stretch, **over):
     for sect, kv in over.items():
         cfg[sect].update(kv)
     return oracle_lib.OracleSolver(cfg, ellipse_mesh(rings, nj, stretch))
-
-
-def _ring_mass_flux(u1, u2, tnx, tny, ni, nj):
-    """Convective mass flux through each interior tau ring i = 1..NI-1 from the state: the face
-    carries (rho u)_f . n with U_f the mean of the two cells (convection.rs:25-40 with the
-    face average of convection.rs:56-66); returns Phi[i] for i = 1..NI-1."""
-    m1 = u1.reshape(ni, nj)
-    m2 = u2.reshape(ni, nj)
-    nx = tnx.reshape(ni + 1, nj)[1:ni]
-    ny = tny.reshape(ni + 1, nj)[1:ni]
-    f = 0.5 * (m1[:-1] + m1[1:]) * nx + 0.5 * (m2[:-1] + m2[1:]) * ny
-    return f.sum(axis=1)
-
-
-def check_mass_conservation(fc0, u1, u2, tnx, tny, ni, nj, rtol):
-    """Discrete conservation, ring by ring: the n faces cancel around each (periodic) ring, so
-    the mass residual summed over rows 0..i-1 equals the flux through ring i minus the flux
-    through the wall.  With the wall flux zero (mirror ghost) every partial sum equals the
-    ring flux Phi_i computed from the state; a sign slip in the face assembly of any row, or
-    mass leaking through the wall, breaks it."""
-    fc = np.asarray(fc0).reshape(ni, nj)
-    cum = np.cumsum(fc.sum(axis=1))[: ni - 1]  # rows 0..i-1 for i = 1..NI-1
-    phi = _ring_mass_flux(u1, u2, tnx, tny, ni, nj)
-    scale = float(np.abs(fc).sum())
-    err = np.abs(cum - phi)
-    assert float(err.max()) <= rtol * scale, (float(err.max()), scale, int(np.argmax(err)))
 
 
 def test_results_are_independent_of_thread_count(oracle_lib):
@@ -99,7 +73,7 @@ def test_discrete_mass_conservation(oracle_lib):
     check_mass_conservation(fc0, s.get("U1"), s.get("U2"), s.get("tau_nx"), s.get("tau_ny"),
                             ni, nj, 1e-13)
     # the oracle's interior ring fluxes are the central mass flux of the state
-    phi = _ring_mass_flux(s.get("U1"), s.get("U2"), s.get("tau_nx"), s.get("tau_ny"), ni, nj)
+    phi = ring_mass_flux(s.get("U1"), s.get("U2"), s.get("tau_nx"), s.get("tau_ny"), ni, nj)
     np.testing.assert_allclose(tf[1:ni].sum(axis=1), phi, rtol=0, atol=1e-13 * np.abs(tf).sum())
 
 
@@ -147,16 +121,6 @@ def test_converges_on_a_moderate_mesh(oracle_lib):
 
 # ---- extensions (no reference counterpart) ----
 
-def _rotate_mesh(text: str, angle_rad: float) -> str:
-    lines = text.splitlines()
-    ca, sa = math.cos(angle_rad), math.sin(angle_rad)
-    out = [lines[0]]
-    for ln in lines[1:]:
-        x, y = (float(v) for v in ln.split())
-        out.append(f"{ca * x - sa * y:.17g} {sa * x + ca * y:.17g}")
-    return "\n".join(out) + "\n"
-
-
 @pytest.mark.parametrize("aoa", [5.0, -3.0])
 def test_forces_are_invariant_under_rotation(oracle_lib, aoa):
     """CL/CD at AoA alpha on a mesh = CL/CD at AoA 0 on the mesh rotated by -alpha (the
@@ -167,7 +131,7 @@ def test_forces_are_invariant_under_rotation(oracle_lib, aoa):
     cfg_b = config_dict()
     cfg_b["simulation"]["AOA"] = 0.0
     a = oracle_lib.OracleSolver(cfg_a, mesh)
-    b = oracle_lib.OracleSolver(cfg_b, _rotate_mesh(mesh, -math.radians(aoa)))
+    b = oracle_lib.OracleSolver(cfg_b, rotate_mesh(mesh, -math.radians(aoa)))
     a.run(20)
     b.run(20)
     fa, fb = a.forces(), b.forces()
