@@ -75,11 +75,11 @@ def _cpu_model() -> str:
 
 
 def _cpu_share() -> int:
-    """Threads this process may use: the affinity mask, capped by OMP_NUM_THREADS when set
-    (the GPU host grants one GPU's share of its cores)."""
-    n = len(os.sched_getaffinity(0))
+    """Threads this process may use: OMP_NUM_THREADS when set (a GPU host grants each GPU a
+    share of its cores and says so there), else every core of the machine.  The calling
+    thread's affinity mask is no guide: the GPU runtime may have narrowed it."""
     env = os.environ.get("OMP_NUM_THREADS", "")
-    return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
 
 
 def cpu_baseline(cfg: str, mesh: str, name: str, budget_s: float = 4.0) -> dict:
@@ -118,6 +118,7 @@ def cpu_baseline(cfg: str, mesh: str, name: str, budget_s: float = 4.0) -> dict:
             "cores": best["threads"], "kind": "port", "cpu_model": _cpu_model(),
             "ms_per_step": best["ms_per_step"], "ns_per_cell_step": best["ns_per_cell_step"],
             "runs": runs, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "host_cpus": os.cpu_count(), "main_thread_affinity": len(os.sched_getaffinity(0)),
             "sample": f"{name} ({cells} cells), one case of the batch: median of 3 runs of "
                       f"the C restatement of jst-rs (FP64, unfused kernels, OpenMP row loops) at "
                       f"{runs['all_cores']['threads']} threads (this process's cores) and "
