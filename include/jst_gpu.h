@@ -213,6 +213,14 @@ jst_status jst_batch_wall(jst_batch *b, int case_id, double *x, double *y, doubl
 jst_status jst_batch_get_field(jst_batch *b, int case_id, const char *name, double *dst,
                                size_t n);
 jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out);
+/* No counterpart in the reference.  For callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc): jst_batch_residual_terms without any
+ * allocation.  The case's work-item list goes into a fixed device array of the library
+ * (65536 items, 1 MiB per device, held until the process ends) instead of a fresh buffer; a
+ * case needing more items fails with JST_ERR_ARG.  Call it with the batch's device current.
+ * Results are bitwise those of jst_batch_residual_terms.  The array is shared by every call
+ * on one device: calls that overlap on one device (two host threads, two batches) race. */
+jst_status jst_batch_residual_terms_noalloc(jst_batch *b, int case_id, double *dt_out);
 /* Residual history of a case: res[k] for steps first..first+n-1 (1-based), within the last
  * 4096 steps. */
 jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, double *res);

@@ -213,6 +213,8 @@ struct jst_batch {
     CaseDev *d_view = nullptr; // FP64 view of one case for the wall-force kernel
     hipEvent_t ev[2] = {nullptr, nullptr};
     std::vector<int> timing_slots; // cases whose stops jst_batch_time_begin switched off
+    int device = 0;                // the HIP device the batch was created on
+    TileDesc *noalloc_items = nullptr; // that device's fixed work-item array (no allocation)
 
     ~jst_batch()
     {
@@ -590,6 +592,10 @@ jst_status jst_batch_create(jst_precision prec, size_t hbm_bytes, int max_cases,
     b->prec = prec;
     b->max_cases = max_cases;
     b->cases.resize(max_cases);
+    HIPCHK(hipGetDevice(&b->device));
+    // set-up of the allocation-free entry points: the device address of this device's fixed
+    // work-item array (looked up once per device)
+    HIPCHK(jst::noalloc_scratch(b->device, &b->noalloc_items));
     HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreate(&b->ev[0]));
     HIPCHK(hipEventCreate(&b->ev[1]));
@@ -871,6 +877,35 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out)
     HIPCHK(hipMemcpyAsync(dt, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice, b->stream));
     HIPCHK(jst::launch_stage_mixed(b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true, b->stream));
     jst_status st = fetch_ctl(b); // synchronises the stream before the list is freed
+    if (st != JST_OK) return st;
+    if (dt_out) *dt_out = b->h_ctl[case_id].dt;
+    return JST_OK;
+}
+
+// The same export without any device allocation: the case's work-item list is generated on
+// the device into the library's fixed per-device array instead of a fresh buffer.  Same
+// kernels, same items (the case's segment length under the batch's tiling), so the exported
+// terms are bitwise those of jst_batch_residual_terms.
+jst_status jst_batch_residual_terms_noalloc(jst_batch *b, int case_id, double *dt_out)
+{
+    HostCase *hc = get_case(b, case_id);
+    if (!hc) return fail(JST_ERR_ARG, "no such case");
+    const int rows = case_rows(b, *hc);
+    const long nitems = (long)((hc->ni + rows - 1) / rows) * ((hc->nj + MIX_COLS - 1) / MIX_COLS);
+    if (nitems > JST_NOALLOC_MAX_ITEMS) {
+        char buf[200];
+        snprintf(buf, sizeof buf,
+                 "case needs %ld work items; the allocation-free export holds at most %d", nitems,
+                 JST_NOALLOC_MAX_ITEMS);
+        return fail(JST_ERR_ARG, buf);
+    }
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != b->device) return fail(JST_ERR_ARG, "the batch's device is not the current device");
+    int n = 0;
+    HIPCHK(jst::launch_make_items(b->noalloc_items, hc->slot, hc->ni, hc->nj, rows, &n, b->stream));
+    HIPCHK(jst::launch_stage_mixed(b->d_cases, b->noalloc_items, n, 0, 0, RK_ALPHA[0], true, b->stream));
+    jst_status st = fetch_ctl(b); // waits for the stream
     if (st != JST_OK) return st;
     if (dt_out) *dt_out = b->h_ctl[case_id].dt;
     return JST_OK;

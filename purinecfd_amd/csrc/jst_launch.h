@@ -44,6 +44,16 @@ hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int n
 hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s);
 hipError_t launch_dt_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, hipStream_t s);
 hipError_t launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);
+
+// Allocation-free work-item lists: a fixed device array of JST_NOALLOC_MAX_ITEMS work items
+// per device.  noalloc_scratch returns its address on `device` (looked up once per device
+// with that device current); launch_make_items writes one case's items into it (every
+// 58-column group x segment of `rows` rows) and fails with hipErrorInvalidValue when they
+// exceed the array.
+#define JST_NOALLOC_MAX_ITEMS 65536
+hipError_t noalloc_scratch(int device, TileDesc **ptr);
+hipError_t launch_make_items(TileDesc *items, int slot, int ni, int nj, int rows, int *nitems,
+                             hipStream_t s);
 hipError_t launch_shadow(const CaseMix *cases, int ci, size_t n5, hipStream_t s);
 
 } // namespace jst

@@ -16,6 +16,7 @@
 #include <type_traits>
 
 #include <algorithm>
+#include <mutex>
 
 #include "jst_dev.h"
 #include "jst_launch.h"
@@ -893,7 +894,67 @@ __global__ void shadow_kernel(const CaseMix *__restrict__ cases, int ci)
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Work-item list of one case in a fixed device array, for the entry points that allocate
+// nothing.  The array holds JST_NOALLOC_MAX_ITEMS items (1 MiB) on every device the library
+// runs on, for the life of the process; it is shared by every such call on that device, so
+// calls that overlap on one device race.  Each call writes every item it then reads.
+// ---------------------------------------------------------------------------------------
+static __device__ __align__(256) TileDesc scratch[JST_NOALLOC_MAX_ITEMS];
+
+// item k -> (case slot, row segment k % nseg of `rows` rows, column group k / nseg)
+__global__ void make_items_kernel(TileDesc *__restrict__ items, int slot, int ni, int rows, int nseg,
+                                  int nitems)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nitems) return;
+    const int seg = k % nseg, grp = k / nseg;
+    const int i0 = seg * rows;
+    items[k] = TileDesc{slot, i0, grp * MIX_COLS, min(i0 + rows, ni)};
+}
+
+namespace {
+// device address of `scratch` per device, looked up once with that device current
+constexpr int MAX_DEVICES = 64;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+} // namespace
+
 namespace jst {
+
+hipError_t noalloc_scratch(int device, TileDesc **ptr)
+{
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return e;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) {
+            const hipError_t e2 = hipSetDevice(cur);
+            if (e == hipSuccess) e = e2;
+        }
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = (TileDesc *)g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+hipError_t launch_make_items(TileDesc *items, int slot, int ni, int nj, int rows, int *nitems,
+                             hipStream_t s)
+{
+    const int nseg = (ni + rows - 1) / rows, ngroups = (nj + MIX_COLS - 1) / MIX_COLS;
+    const long n = (long)nseg * ngroups;
+    if (n > JST_NOALLOC_MAX_ITEMS) return hipErrorInvalidValue;
+    *nitems = (int)n;
+    hipLaunchKernelGGL(make_items_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, items,
+                       slot, ni, rows, nseg, (int)n);
+    return hipGetLastError();
+}
 
 const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64, 8, 4, 2, 1};
 

@@ -34,7 +34,8 @@ EXPORTS = ["jst_last_error", "jst_config_check", "jst_mesh_parse", "jst_create",
            "jst_wall", "jst_write_wall_csv", "jst_write_csv", "jst_progress", "jst_time_steps",
            "jst_batch_create", "jst_batch_destroy", "jst_case_footprint", "jst_batch_add",
            "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces", "jst_batch_set_state",
-           "jst_batch_wall", "jst_batch_get_field", "jst_batch_residual_terms", "jst_batch_history",
+           "jst_batch_wall", "jst_batch_get_field", "jst_batch_residual_terms",
+           "jst_batch_residual_terms_noalloc", "jst_batch_history",
            "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_begin", "jst_batch_time_end",
            "jst_batch_time_steps", "jst_batch_time_kernels",
            "jst_batch_counts",
@@ -154,6 +155,7 @@ def lib():
         L.jst_batch_wall.argtypes = [vp, ip, dp, dp, dp, dp]
         L.jst_batch_get_field.argtypes = [vp, ip, C.c_char_p, dp, sz]
         L.jst_batch_residual_terms.argtypes = [vp, ip, dp]
+        L.jst_batch_residual_terms_noalloc.argtypes = [vp, ip, dp]
         L.jst_batch_history.argtypes = [vp, ip, C.c_long, C.c_long, dp]
         L.jst_batch_set_tiling.argtypes = [vp, ip, ip]
         L.jst_batch_tune.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
@@ -458,6 +460,12 @@ class Batch:
     def residual_terms(self, cid: int) -> float:
         dt = C.c_double()
         _check(lib().jst_batch_residual_terms(self._h, cid, C.byref(dt)))
+        return dt.value
+
+    def residual_terms_noalloc(self, cid: int) -> float:
+        """residual_terms without any device allocation (fixed per-device work-item array)."""
+        dt = C.c_double()
+        _check(lib().jst_batch_residual_terms_noalloc(self._h, cid, C.byref(dt)))
         return dt.value
 
     def history(self, cid: int, first: int, n: int) -> np.ndarray:
