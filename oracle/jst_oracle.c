@@ -32,6 +32,7 @@ void jo_config_defaults_ext(jo_config *c)
     c->ref_length = 1.0;
     c->local_time_stepping = 0;
     c->residual_drop = 0.0;
+    c->viscous_time_step = 0;
 }
 
 /* config.rs:153-156: mu0 * (r*sqrt r) * (T0+Ts) / (T+Ts) */
@@ -268,6 +269,7 @@ struct jo_solver {
     double *diff; /* x10: x[0..5] y[0..5] */
     double *fc, *fv, *fd; /* x5 */
     double *src, *localdt;
+    double *dtc; /* convective local dt, the JST spectral radius (== localdt unless EXT viscous dt) */
     /* Faces: tau (ni+1) x nj, nrm ni x nj, halo 0 */
     double *tflux, *tdiff, *tdiss, *tlam, *teps;
     double *nflux, *ndiff, *ndiss, *nlam, *neps;
@@ -586,7 +588,25 @@ double jo_timestep(jo_solver *s)
             double u = s->vx[k], v = s->vy[k];
             double conv = fabs(u * a + v * b) + fabs(u * cc + v * dd);
             double acou = s->c[k] * (sqrt(a * a + b * b) + sqrt(cc * cc + dd * dd));
-            s->localdt[k] = cfl * s->vol[k] / (conv + acou);
+            double dtc = cfl * s->vol[k] / (conv + acou);
+            s->dtc[k] = dtc;
+            if (s->cfg.viscous_time_step) {
+                /* EXT: the viscous spectral radius of the cell, both directions (Blazek,
+                 * Computational Fluid Dynamics, 3rd ed., eqs. 6.21-6.22, with C = 4):
+                 * max(4/3, gamma)/rho (mu/Pr + mu_t/Prt) (|A|^2 + |C|^2)/V, mu_t >= 0 */
+                double rho = s->rho[k], t = s->t[k];
+                double mu = jo_mu(&s->cfg, &s->d, t);
+                double rnu = rho * s->nut[k];
+                double chi = rnu / mu, c3 = chi * chi * chi;
+                double mut = rnu * (c3 / (c3 + s->d.cv1_cubed));
+                if (!(mut > 0.0)) mut = 0.0;
+                double g = s->cfg.gamma > 4.0 / 3.0 ? s->cfg.gamma : 4.0 / 3.0;
+                double lv = g / rho * (mu * s->d.inv_pr + mut * s->d.inv_prt) *
+                            (a * a + b * b + cc * cc + dd * dd) / s->vol[k];
+                s->localdt[k] = cfl * s->vol[k] / (conv + acou + 4.0 * lv);
+            } else {
+                s->localdt[k] = dtc;
+            }
         }
     }
     double m = INFINITY;
@@ -813,7 +833,7 @@ void jo_dissipation(jo_solver *s)
     for (i = 0; i < ni; ++i) {
         int jj;
         for (jj = 0; jj < nj; ++jj)
-            s->spec_ratio[CI(s, i, jj)] = s->vol[CI(s, i, jj)] / s->localdt[CI(s, i, jj)];
+            s->spec_ratio[CI(s, i, jj)] = s->vol[CI(s, i, jj)] / s->dtc[CI(s, i, jj)];
     }
 #pragma omp parallel for schedule(static)
     for (i = 0; i <= ni; ++i) {
@@ -1121,7 +1141,7 @@ jo_solver *jo_solver_new(const jo_config *cfg, const jo_mesh *mesh)
     s->nut = zalloc(nc);
     s->grad = zalloc(8 * nc); s->aux = zalloc(3 * nc); s->diff = zalloc(10 * nc);
     s->fc = zalloc(5 * nc); s->fv = zalloc(5 * nc); s->fd = zalloc(5 * nc);
-    s->src = zalloc(nc); s->localdt = zalloc(nc);
+    s->src = zalloc(nc); s->localdt = zalloc(nc); s->dtc = zalloc(nc);
     s->tflux = zalloc(5 * nt); s->tdiff = zalloc(5 * nt); s->tdiss = zalloc(5 * nt);
     s->tlam = zalloc(nt); s->teps = zalloc(2 * nt);
     s->nflux = zalloc(5 * nn); s->ndiff = zalloc(5 * nn); s->ndiss = zalloc(5 * nn);
@@ -1155,7 +1175,7 @@ void jo_solver_free(jo_solver *s)
                      s->nut, s->grad, s->aux, s->diff, s->fc, s->fv, s->fd, s->src, s->localdt,
                      s->tflux, s->tdiff, s->tdiss, s->tlam, s->teps, s->nflux, s->ndiff,
                      s->ndiss, s->nlam, s->neps, s->sensor_i, s->sensor_j, s->spec_ratio,
-                     s->rho_prev};
+                     s->rho_prev, s->dtc};
     size_t k;
     for (k = 0; k < sizeof all / sizeof all[0]; ++k) free(all[k]);
     free(s);

@@ -65,6 +65,7 @@ typedef struct {
     double ref_length;       /* L for Re and CL/CD (default 1) */
     int local_time_stepping; /* 0: global min dt (solver.rs:110,115) */
     double residual_drop;    /* decades below step-1 residual; <= 0: off */
+    int viscous_time_step;   /* 0: convective spectral radius only (timestep.rs:24-47) */
 } jo_config;
 
 /* Derived constants (config.rs:114-146, finish 173-206). */

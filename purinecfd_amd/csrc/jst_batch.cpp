@@ -182,6 +182,7 @@ MixConst make_mix_const(const Config &c, const Derived &d)
     m.inv_kappa2 = (float)d.inv_kappa2;
     m.rmax = (float)c.rmax;
     m.lts = c.lts ? 1 : 0;
+    m.visc_dt = c.visc_dt ? 1 : 0;
     return m;
 }
 

@@ -25,6 +25,7 @@ struct Config {
     // default-off extensions (absent from the reference; defaults reproduce it)
     double nut_ratio = 0.1, reynolds = 0.0, ref_length = 1.0, residual_drop = 0.0;
     bool lts = false;
+    bool visc_dt = false; // simulation.viscous_time_step: viscous spectral radius in the local dt
 };
 
 struct Derived {

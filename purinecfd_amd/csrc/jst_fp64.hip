@@ -168,6 +168,23 @@ __device__ __forceinline__ double pressure_sensor(double pm, double p0, double p
     return fabs((pm - 2.0 * p0 + pp) / (pm + 2.0 * p0 + pp));
 }
 
+// Viscous spectral radius of a cell over both directions (extension simulation.viscous_time_step;
+// Blazek, Computational Fluid Dynamics, 3rd ed., eqs. 6.21-6.22): max(4/3, gamma)/rho
+// (mu/Pr + mu_t/Prt) (|A|^2 + |C|^2)/V with (a, b), (c, d) the mean face normals of the
+// convective radius and mu_t >= 0.  The local dt becomes CFL V / (conv + acou + 4 visc); the
+// JST spectral radius keeps the convective part only.
+__device__ __forceinline__ double visc_radius(const PhysConst &k, double rho, double t, double rho_nu,
+                                              double a, double b, double c, double d, double vol)
+{
+    const double r = t * k.inv_t_ref;
+    const double mu = r * sqrt(r) * k.suth_num / (t + k.t_suth); // config.rs:210-213
+    const double chi = rho_nu / mu, c3 = chi * chi * chi;
+    double mut = rho_nu * (c3 / (c3 + k.cv1_cubed));
+    if (!(mut > 0.0)) mut = 0.0;
+    const double g = k.gamma > 4.0 / 3.0 ? k.gamma : 4.0 / 3.0;
+    return g / rho * (mu * k.inv_pr + mut * k.inv_prt) * (a * a + b * b + c * c + d * d) / vol;
+}
+
 // jst-rs viscous.rs:37-71 (diffusion_tensor).  d[0..3] = D_x[1..4], d[4..7] = D_y[1..4]
 // (component 0 of both columns is identically zero).
 __device__ __forceinline__ void diffusion_tensor(const PhysConst &k, double t, double rho_nu,
@@ -600,7 +617,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
         const size_t cc = cidx(i, j, pitch);
         const double a_dt = k.lts ? alpha * C.localdt[cur + cc] : alpha * dt;
         const double scl = a_dt * C.inv_vol[cc];
-        double un[4], rho_n = 0.0;
+        double un[5], rho_n = 0.0;
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
             double fl[4], vf[4], df[4];
@@ -645,7 +662,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
                 const double fo = former[m * plane + cc];
                 const double v = fo - r * scl;
                 dst[m * plane + cc] = v;
-                if (m < 4) un[m] = v;
+                un[m] = v;
                 if (m == 0) rho_n = fo;
             }
         }
@@ -679,9 +696,12 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
                 const double conv = fabs(vx * aa + vy * bb) + fabs(vx * ccv + vy * ddv);
                 const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(ccv * ccv + ddv * ddv));
                 const double vol = C.vol[cc];
-                const double ldt = k.cfl * vol / (conv + acou);
+                const double dtc = k.cfl * vol / (conv + acou);
+                const double ldt = k.visc_dt ? k.cfl * vol / (conv + acou + 4.0 * visc_radius(k, un[0], t,
+                                                                un[0] * (un[4] * inv), aa, bb, ccv, ddv, vol))
+                                             : dtc;
                 C.localdt[nxt + cc] = ldt;
-                C.ratio[nxt + cc] = vol / ldt;
+                C.ratio[nxt + cc] = vol / dtc;
                 dmin = fmin(dmin, ldt);
             }
         }
@@ -754,9 +774,12 @@ __global__ __launch_bounds__(NT) void dt_tile_kernel(const CaseDev *__restrict__
         const double ddv = 0.5 * (C.nny[c] + C.nny[cr]);
         const double conv = fabs(q.vx * aa + q.vy * bb) + fabs(q.vx * ccv + q.vy * ddv);
         const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(ccv * ccv + ddv * ddv));
-        const double ldt = k.cfl * C.vol[c] / (conv + acou);
+        const double dtc = k.cfl * C.vol[c] / (conv + acou);
+        const double ldt = k.visc_dt ? k.cfl * C.vol[c] / (conv + acou + 4.0 * visc_radius(k, q.rho, q.t,
+                                                            q.rho * q.nut, aa, bb, ccv, ddv, C.vol[c]))
+                                     : dtc;
         C.localdt[cur + c] = ldt;
-        C.ratio[cur + c] = C.vol[c] / ldt;
+        C.ratio[cur + c] = C.vol[c] / dtc;
         dmin = fmin(dmin, ldt);
     }
     red_m[tid] = dmin;

@@ -318,6 +318,9 @@ jst_status parse_config(const char *text, Config &c)
         const JVal *v = sim->get("local_time_stepping");
         if (v && v->kind == JVal::BOOL) c.lts = v->b;
         else if (v && err.empty()) err = "invalid type for `local_time_stepping`: expected bool";
+        const JVal *w = sim->get("viscous_time_step");
+        if (w && w->kind == JVal::BOOL) c.visc_dt = w->b;
+        else if (w && err.empty()) err = "invalid type for `viscous_time_step`: expected bool";
     }
     const JVal *sol = root.get("solver");
     if (sol && sol->kind == JVal::OBJ) {
@@ -625,6 +628,7 @@ PhysConst make_const(const Config &c, const Derived &d)
     k.residual_drop = c.residual_drop;
     k.supersonic = d.supersonic ? 1 : 0;
     k.lts = c.lts ? 1 : 0;
+    k.visc_dt = c.visc_dt ? 1 : 0;
     return k;
 }
 

@@ -230,6 +230,34 @@ __device__ __forceinline__ float sa_source32(const MixConst &k, float drho, floa
     return (prod - destr + diff) * vol;
 }
 
+// Viscous spectral radius of a cell (extension simulation.viscous_time_step; Blazek,
+// Computational Fluid Dynamics, 3rd ed., eqs. 6.21-6.22): max(4/3, gamma)/rho (mu/Pr +
+// mu_t/Prt) (|A|^2 + |C|^2)/V, mu_t >= 0.  FP32 from the new state: rho, 1/rho, p, rho nu~.
+__device__ __forceinline__ float visc_radius32(const MixConst &k, float rho, float inv, float p, float rnu,
+                                               float a, float b, float c, float d, float vol)
+{
+    const float t = p * inv * (1.0f / k.r_gas);
+    const float r = t * k.inv_t_ref;
+    const float mu = r * fsqrt(r) * k.suth_num * frcp(t + k.t_suth);
+    const float chi = rnu * frcp(mu), c3 = chi * chi * chi;
+    const float mut = fmaxf(rnu * c3 * frcp(c3 + k.cv1_cubed), 0.0f);
+    const float g = k.gamma > 4.0f / 3.0f ? k.gamma : 4.0f / 3.0f;
+    return g * inv * (mu * k.inv_pr + mut * k.inv_prt) * (a * a + b * b + c * c + d * d) * frcp(vol);
+}
+
+// The same in FP64 (dt set-up from the FP64 state), in the operation order of the plain path.
+__device__ __forceinline__ double visc_radius64(const PhysConst &k, double rho, double t, double rho_nu,
+                                                double a, double b, double c, double d, double vol)
+{
+    const double r = t * k.inv_t_ref;
+    const double mu = r * sqrt(r) * k.suth_num / (t + k.t_suth);
+    const double chi = rho_nu / mu, c3 = chi * chi * chi;
+    double mut = rho_nu * (c3 / (c3 + k.cv1_cubed));
+    if (!(mut > 0.0)) mut = 0.0;
+    const double g = k.gamma > 4.0 / 3.0 ? k.gamma : 4.0 / 3.0;
+    return g / rho * (mu * k.inv_pr + mut * k.inv_prt) * (a * a + b * b + c * c + d * d) / vol;
+}
+
 __device__ __forceinline__ float sensor32(float pinf4, float dm, float d0, float dp)
 {
     // |p- - 2 p0 + p+| / (p- + 2 p0 + p+) with p = p_inf + dp (jst-rs dissipation.rs:29-31)
@@ -659,7 +687,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     const float cs = fsqrt(k.gamma * fmaxf(pr, 0.0f) * inv);
                     const float conv = fabsf(vx * aa + vy * bb) + fabsf(vx * cc + vy * dd);
                     const float acou = cs * (fsqrt(aa * aa + bb * bb) + fsqrt(cc * cc + dd * dd));
-                    const float ldt = k.cfl * VOL[0] / (conv + acou);
+                    float lam = conv + acou;
+                    if (k.visc_dt) // wave-uniform: the viscous spectral radius joins the local dt
+                        lam += 4.0f * visc_radius32(k, rho, inv, pr, (float)u1[4], aa, bb, cc, dd, VOL[0]);
+                    const float ldt = k.cfl * VOL[0] / lam;
                     const double dr = u1[0] - former[0];
                     // this column's residual sum and dt minimum over a 2^bshift-row block
                     racc += own ? dr * dr : 0.0;
@@ -848,9 +879,15 @@ __global__ __launch_bounds__(NT) void mix_dt_kernel(const CaseMix *__restrict__ 
         const double conv = fabs(vx * aa + vy * bb) + fabs(vx * ccv + vy * ddv);
         const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(ccv * ccv + ddv * ddv));
         const double vol = (double)C.vol[c];
-        const double ldt = k.cfl * vol / (conv + acou);
+        const double dtc = k.cfl * vol / (conv + acou);
+        double ldt = dtc;
+        if (k.visc_dt) {
+            const double t = p / (k.r_gas * rho);
+            ldt = k.cfl * vol / (conv + acou + 4.0 * visc_radius64(k, rho, t, rho * (u[4 * plane + c] * inv),
+                                                                   aa, bb, ccv, ddv, vol));
+        }
         C.localdt[rot][c] = (float)ldt;
-        C.ratio[rot][c] = (float)(vol / ldt);
+        C.ratio[rot][c] = (float)(vol / dtc);
         dmin = fmin(dmin, ldt);
     }
     red_m[tid] = dmin;

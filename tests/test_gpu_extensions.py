@@ -66,3 +66,43 @@ def test_residual_drop_stops_where_the_oracle_does(jst, oracle_lib, precision):
         assert rep.steps == orep["steps"]
     else:  # the drop is crossed within a step or two of the FP64 step
         assert abs(rep.steps - orep["steps"]) <= 2
+
+
+def _visc_case():
+    cfg = config_dict()
+    cfg["simulation"].update({"AOA": 2.0, "Ma": 0.4, "CFL": 1.0, "local_time_stepping": True,
+                              "viscous_time_step": True})
+    cfg["physics"]["mu0"] *= 1e3  # viscous enough that the viscous radius moves the local dt
+    return cfg, ellipse_mesh(25, 64, 1.1)
+
+
+def test_viscous_time_step_fp64_matches_oracle(jst, oracle_lib):
+    cfg, text = _visc_case()
+    s = jst.Solver(json.dumps(cfg), text)
+    o = oracle_lib.OracleSolver(cfg, text)
+    hist = []
+    s.run(60, on_step=lambda k, r: hist.append(r))
+    orep = o.run(60)
+    field_cmp(np.array(hist), orep["history"], "residual history", 1e-9)
+    for name in ("rho", "p", "u", "v", "nut"):
+        field_cmp(s.get(name), o.get(name), name, 1e-9, 1e-300)
+    o.timestep()
+    field_cmp(s.get("localdt"), o.get("localdt"), "localdt", 1e-9)
+    conv = oracle_lib.OracleSolver({**cfg, "simulation": {**cfg["simulation"], "viscous_time_step": False}}, text)
+    conv.set_state(np.stack([o.get(f"U{k}") for k in range(5)]))
+    conv.timestep()
+    assert np.all(o.get("localdt") < conv.get("localdt"))  # the viscous radius is active
+
+
+def test_viscous_time_step_mixed_within_fp32_level(jst, oracle_lib):
+    cfg, text = _visc_case()
+    s = jst.Solver(json.dumps(cfg), text, jst.MIXED)
+    o = oracle_lib.OracleSolver(cfg, text)
+    s.run(60)
+    o.run(60)
+    for name in ("rho", "p", "u", "v"):
+        a, b = s.get(name), o.get(name)
+        assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-5, name
+    o.timestep()
+    a, b = s.get("localdt"), o.get("localdt")
+    assert np.max(np.abs(a - b) / b) <= 1e-4

@@ -264,3 +264,59 @@ def test_halo_depth_beyond_three_changes_nothing(oracle_lib):
     b.run(20)
     for name in ("rho", "u", "v", "p", "nut"):
         assert np.array_equal(a.get(name), b.get(name)), name
+
+
+# ---- viscous time step (extension simulation.viscous_time_step) ----
+
+def test_viscous_time_step_keeps_a_viscous_case_stable(oracle_lib):
+    """At very low Reynolds number the explicit viscous limit, not the acoustic one, bounds
+    the step: the reference's convective-only dt diverges within a few steps, the dt with the
+    viscous spectral radius stays physical."""
+    over = {"physics": {"mu0": config_dict()["physics"]["mu0"] * 1e7}}
+    a = _solver(oracle_lib, 17, 48, 1.1, **over)
+    with pytest.raises(oracle_lib.NonPhysical):
+        a.run(300)
+    b = _solver(oracle_lib, 17, 48, 1.1, simulation={"viscous_time_step": True}, **over)
+    rep = b.run(300)
+    assert rep["steps"] == 300 and (b.get("rho") > 0).all() and (b.get("p") > 0).all()
+
+
+def test_viscous_time_step_vanishes_without_viscosity(oracle_lib):
+    """The viscous radius is additive: as mu -> 0 the local dt tends to the convective one,
+    and it is never larger; the JST spectral radius (dissipation) is the convective one in
+    both cases, so Fd is bitwise unchanged on the same state."""
+    for mu_scale, rtol in ((1e-7, 1e-9), (1.0, None)):
+        over = {"physics": {"mu0": config_dict()["physics"]["mu0"] * mu_scale}}
+        a = _solver(oracle_lib, 13, 40, 1.2, **over)
+        b = _solver(oracle_lib, 13, 40, 1.2, simulation={"viscous_time_step": True}, **over)
+        a.run(10)
+        u = np.stack([a.get(f"U{k}") for k in range(5)])
+        b.set_state(u)
+        a.timestep()
+        b.timestep()
+        la, lb = a.get("localdt"), b.get("localdt")
+        assert (lb <= la).all()
+        if rtol is not None:
+            np.testing.assert_allclose(lb, la, rtol=rtol)
+        else:
+            assert (lb < la).all()
+        a.residual_terms()
+        b.residual_terms()
+        for k in range(5):
+            assert np.array_equal(a.get(f"Fd{k}"), b.get(f"Fd{k}"))
+
+
+def test_viscous_local_time_stepping_reaches_the_global_dt_steady_state(oracle_lib):
+    mesh = ellipse_mesh(9, 32, 1.0)
+    runs = []
+    for sim in ({}, {"local_time_stepping": True, "viscous_time_step": True}):
+        cfg = config_dict()
+        cfg["simulation"].update(sim)
+        cfg["solver"]["targetres"] = 1e-12
+        s = oracle_lib.OracleSolver(cfg, mesh)
+        rep = s.run(40000)
+        assert rep["converged"], (sim, rep["residual"])
+        runs.append(s)
+    for name in ("rho", "u", "v", "p"):
+        a, b = runs[0].get(name), runs[1].get(name)
+        assert float(np.abs(a - b).max()) <= 1e-7 * float(np.abs(a).max()), name
