@@ -226,10 +226,11 @@ jst_status jst_batch_residual_terms_noalloc(jst_batch *b, int case_id, double *d
 jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, double *res);
 /* Work decomposition of the stage kernel: variant 0..9 = row-segment length (16, 24, 32, 40,
  * 48, 64, 8, 4, 2 or 1 rows per wave work item of 58 columns) for every case, variant 10 =
- * each case its own length from its size alone (the shortest of 1, 2, 4 rows whose doubled
- * length still leaves fewer work items than 1024, else 8); grid = workgroups of 4 waves
- * (0 = one per 4 work items).  The grid never changes results; the segment length changes
- * them only at FP32 rounding level (where a segment's edge rows are evaluated). */
+ * each case its own length from its size alone (the shortest whose work items number at
+ * most 2048, the waves the GPU keeps resident), variant 100 + L (1 <= L <= 256)
+ * = L rows for every case; grid = workgroups of 4 waves (0 = one per 4 work items).  The
+ * grid never changes results; the segment length changes them only at FP32 rounding level
+ * (where a segment's edge rows are evaluated). */
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);
 /* Autotuner: times every variant with three grid sizes (0, 512, 1024) on the current batch
  * for trial_steps steps each, in two rounds (each candidate keeps its faster time; the steps
@@ -283,7 +284,7 @@ typedef struct {
                                  is saturated (fewer halo rows, for throughput); a switch
                                  moves results at FP32 rounding level.  -1 = each case its
                                  own length always (results bitwise independent of the
-                                 load).  1, 2, 4, 8, 16, ... = that length for every case */
+                                 load).  1 .. 256 = that length for every case */
 } jst_serve_opts;
 
 typedef struct {

@@ -254,16 +254,18 @@ jst_status drop_graph(jst_batch *b)
 // one writer)
 int block_shift_for(int rows) { return rows % 4 == 0 ? 2 : (rows % 2 == 0 ? 1 : 0); }
 
-// A case's own row-segment length, from its size alone: the shortest of 1, 2 and 4 rows whose
-// doubled length would still leave fewer work items than the GPU has SIMDs (1024), else 8
-// rows.  Short segments give a small case the most waves (lowest latency while the GPU is
-// not full); longer ones re-read fewer halo rows.
+// A case's own row-segment length, from its size alone: the shortest segment whose work items
+// (58-column groups x segments) all fit in the 2048 waves the GPU keeps resident at once (two
+// per SIMD).  Short segments give a case the most waves (lowest latency while the GPU is not
+// full); a longer one would only re-read fewer halo rows, a shorter one leaves waves queued
+// behind the resident ones (the 1024 x 512-cell case: 172 / 147 / 185 us per step at 8 / 5 /
+// 4 rows).
 int own_segment_rows(int ni, int nj)
 {
     const long groups = (nj + MIX_COLS - 1) / MIX_COLS;
-    for (int rows : {1, 2, 4})
-        if (groups * ((ni + 2 * rows - 1) / (2 * rows)) < 1024) return rows;
-    return 8;
+    for (int rows = 1; rows < MIX_MAX_ROWS; ++rows)
+        if (groups * ((ni + rows - 1) / rows) <= 2048) return rows;
+    return MIX_MAX_ROWS;
 }
 
 // Row-segment length of a case under the batch's tiling: the variant's length, or under
@@ -272,6 +274,7 @@ int case_rows(const jst_batch *b, const HostCase &hc);
 
 int case_rows(const jst_batch *b, const HostCase &hc)
 {
+    if (b->variant >= MIX_VARIANT_ROWS) return b->variant - MIX_VARIANT_ROWS;
     return b->variant < MIX_NVARIANTS ? jst::MIX_SEGMENT_ROWS[b->variant] : own_segment_rows(hc.ni, hc.nj);
 }
 
@@ -984,7 +987,9 @@ jst_status jst_batch_get_field(jst_batch *b, int case_id, const char *name, doub
 
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid)
 {
-    if (!b || variant < 0 || variant > MIX_VARIANT_PER_CASE || grid < 0) return fail(JST_ERR_ARG, "bad tiling");
+    const bool rows_ok = variant > MIX_VARIANT_ROWS && variant <= MIX_VARIANT_ROWS + MIX_MAX_ROWS;
+    if (!b || variant < 0 || (variant > MIX_VARIANT_PER_CASE && !rows_ok) || grid < 0)
+        return fail(JST_ERR_ARG, "bad tiling");
     HIPCHK(hipStreamSynchronize(b->stream));
     if (variant != b->variant) b->dirty = true;
     if (grid != b->grid) {
@@ -1196,8 +1201,7 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     const bool adaptive = opts->segment_rows == 0;
     {
         int v = opts->segment_rows <= 0 ? MIX_VARIANT_PER_CASE : -1;
-        for (int q = 0; q < MIX_NVARIANTS; ++q)
-            if (jst::MIX_SEGMENT_ROWS[q] == opts->segment_rows) v = q;
+        if (opts->segment_rows >= 1 && opts->segment_rows <= MIX_MAX_ROWS) v = MIX_VARIANT_ROWS + opts->segment_rows;
         if (v < 0) return fail(JST_ERR_ARG, "unsupported segment_rows");
         if ((st = jst_batch_set_tiling(b, v, 0)) != JST_OK) return st;
     }

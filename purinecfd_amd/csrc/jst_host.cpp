@@ -878,9 +878,9 @@ jst_status jst_create(const char *config_json, const char *mesh_text, jst_precis
         if (sm != JST_OK) return sm;
         if ((sm = jst_batch_create(JST_MIXED, bytes + (4u << 20), 1, &x->mix)) != JST_OK) return sm;
         // one case alone takes its own row-segment length, chosen from its size: the
-        // shortest of 1, 2, 4 rows whose doubled length still leaves fewer work items than the
-        // GPU has SIMDs, else 8.  Measured: 256x128 cells 67 / 77 / 91 / 120 us per step with
-        // 1 / 2 / 4 / 8 rows; 1024x512 cells 272 / 194 / 173 / 163 us.
+        // shortest whose work items all fit in the 2048 resident waves.  Measured: 256x128
+        // cells 67 / 77 / 91 / 120 us per step with 1 / 2 / 4 / 8 rows (1 is chosen);
+        // 1024x512 cells 324 / 208 / 185 / 147 / 172 us with 1 / 2 / 4 / 5 / 8 rows (5).
         if ((sm = jst_batch_set_tiling(x->mix, MIX_VARIANT_PER_CASE, 0)) != JST_OK) return sm;
         if ((sm = jst_batch_add(x->mix, config_json, mesh_text, 0, &x->mix_id)) != JST_OK) return sm;
         x->d = finish(x->cfg);

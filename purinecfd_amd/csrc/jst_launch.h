@@ -37,6 +37,9 @@ hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, dou
 // general row body), never on the launch grid.
 #define MIX_NVARIANTS 10
 #define MIX_VARIANT_PER_CASE MIX_NVARIANTS
+// variant MIX_VARIANT_ROWS + L (1 <= L <= MIX_MAX_ROWS): every case L rows per work item
+#define MIX_VARIANT_ROWS 100
+#define MIX_MAX_ROWS 256
 #define MIX_COLS 58
 extern const int MIX_SEGMENT_ROWS[MIX_NVARIANTS];
 hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,

@@ -115,8 +115,9 @@ def c5_requests(seconds: float = 60.0, rate: float = 20.0, seed: int = 20261204,
                 cylinder_fraction: float = 0.5) -> list[dict]:
     """Serving stream: Poisson arrivals (mean `rate` solves/s) over `seconds`; each request is
     a cylinder case (40x17, 80x33 or 160x65 points, M 0.2, Re 1e3, CFL 2, converged at a
-    5-decade residual drop, at most 4000 steps) with probability `cylinder_fraction`, else a
-    draw from the C4 distribution (at most 800 steps, 6-decade drop).
+    6-decade residual drop, which takes about 2900, 6100 and 10000 steps, at most 12000
+    steps) with probability `cylinder_fraction`, else a draw from the C4 distribution (at
+    most 800 steps, 6-decade drop).
 
     The C4 budget stays below the first divergence seen in that distribution: with the
     reference's global time step (convective spectral radius only) about half of its cases
@@ -133,8 +134,8 @@ def c5_requests(seconds: float = 60.0, rate: float = 20.0, seed: int = 20261204,
             pts, rings = CYLINDER_SIZES[rng.randrange(len(CYLINDER_SIZES))]
             mesh, _ = meshgen.cylinder_mesh(pts, rings, 0.5, 20.0)
             cfg = make_config(mach=0.2, aoa=0.0, cfl=2.0, reynolds=1e3, nut_ratio=3.0,
-                              iteration=4000, residual_drop=5.0)
-            out.append({"arrival_s": t, "config": cfg, "mesh": mesh, "max_steps": 4000,
+                              iteration=12000, residual_drop=6.0)
+            out.append({"arrival_s": t, "config": cfg, "mesh": mesh, "max_steps": 12000,
                         "name": f"cyl_{pts}x{rings}"})
         else:
             w = pool[rng.randrange(len(pool))]
