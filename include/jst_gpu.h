@@ -153,6 +153,14 @@ jst_status jst_progress(const jst_ctx *ctx, long *step, double *totaltime);
  * stage_ms receives the mean stage-kernel duration.  The device does not stop on
  * convergence during timing. */
 jst_status jst_time_steps(jst_ctx *ctx, int nsteps, double *total_ms, double *stage_ms);
+/* Timing hooks that let the caller time steps with its own clock (no reference counterpart):
+ * jst_time_begin switches off the convergence and step-limit stops and does every set-up the
+ * steps need (on the FP32-mixed path: launch lists and HIP graphs built and launched once),
+ * then waits; jst_time_run queues exactly nsteps steps and waits for them; jst_time_end
+ * restores the stops and reports a non-physical state reached meanwhile. */
+jst_status jst_time_begin(jst_ctx *ctx, int nsteps);
+jst_status jst_time_run(jst_ctx *ctx, int nsteps);
+jst_status jst_time_end(jst_ctx *ctx);
 
 
 /* ======================================================================================

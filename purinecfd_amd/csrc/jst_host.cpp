@@ -1378,4 +1378,37 @@ jst_status jst_time_steps(jst_ctx *x, int nsteps, double *total_ms, double *stag
     return JST_OK;
 }
 
+
+jst_status jst_time_begin(jst_ctx *x, int nsteps)
+{
+    if (!x || nsteps <= 0) return fail(JST_ERR_ARG, "bad arguments");
+    if (x->mix) return jst_batch_time_begin(x->mix, nsteps);
+    jst_status st;
+    if ((st = ctl_set(x, &CaseCtl::timing, 1)) != JST_OK) return st;
+    if ((st = ctl_set(x, &CaseCtl::status, (int)JST_RUN_ACTIVE)) != JST_OK) return st;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return JST_OK;
+}
+
+jst_status jst_time_run(jst_ctx *x, int nsteps)
+{
+    if (!x || nsteps <= 0) return fail(JST_ERR_ARG, "bad arguments");
+    if (x->mix) return jst_batch_step(x->mix, nsteps, 1);
+    jst_status st = launch_steps(x, nsteps);
+    if (st != JST_OK) return st;
+    HIPCHK(hipStreamSynchronize(x->stream));
+    return JST_OK;
+}
+
+jst_status jst_time_end(jst_ctx *x)
+{
+    if (!x) return fail(JST_ERR_ARG, "bad arguments");
+    if (x->mix) return jst_batch_time_end(x->mix);
+    jst_status st;
+    if ((st = ctl_set(x, &CaseCtl::timing, 0)) != JST_OK) return st;
+    if ((st = ctl_fetch(x)) != JST_OK) return st;
+    if (x->h_ctl->err_key != JST_ERR_NONE) return report_nonphysical(x, nullptr);
+    return JST_OK;
+}
+
 } // extern "C"

@@ -32,6 +32,7 @@ EXPORTS = ["jst_last_error", "jst_config_check", "jst_mesh_parse", "jst_create",
            "jst_create_from_paths", "jst_destroy", "jst_dims", "jst_advance", "jst_run",
            "jst_residual", "jst_residual_terms", "jst_get_field", "jst_set_state", "jst_forces",
            "jst_wall", "jst_write_wall_csv", "jst_write_csv", "jst_progress", "jst_time_steps",
+           "jst_time_begin", "jst_time_run", "jst_time_end",
            "jst_batch_create", "jst_batch_destroy", "jst_case_footprint", "jst_batch_add",
            "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces", "jst_batch_set_state",
            "jst_batch_wall", "jst_batch_get_field", "jst_batch_residual_terms",
@@ -141,6 +142,9 @@ def lib():
         L.jst_write_wall_csv.argtypes = [vp, C.c_char_p]
         L.jst_progress.argtypes = [vp, C.POINTER(C.c_long), dp]
         L.jst_time_steps.argtypes = [vp, ip, dp, dp]
+        L.jst_time_begin.argtypes = [vp, ip]
+        L.jst_time_run.argtypes = [vp, ip]
+        L.jst_time_end.argtypes = [vp]
         sz = C.c_size_t
         L.jst_batch_create.argtypes = [ip, sz, ip, C.POINTER(vp)]
         L.jst_batch_destroy.argtypes = [vp]
@@ -371,6 +375,17 @@ class Solver:
         s, t = C.c_long(), C.c_double()
         _check(lib().jst_progress(self._h, C.byref(s), C.byref(t)))
         return t.value
+
+    def time_begin(self, nsteps: int) -> None:
+        """Untimed set-up of nsteps timed steps (stops off, graphs built and warmed)."""
+        _check(lib().jst_time_begin(self._h, int(nsteps)))
+
+    def time_run(self, nsteps: int) -> None:
+        """Exactly nsteps steps, waited for (between time_begin and time_end)."""
+        _check(lib().jst_time_run(self._h, int(nsteps)))
+
+    def time_end(self) -> None:
+        _check(lib().jst_time_end(self._h))
 
     def time_steps(self, nsteps: int, per_stage: bool = False):
         total, stage = C.c_double(), C.c_double()
