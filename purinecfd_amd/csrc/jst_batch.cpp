@@ -278,17 +278,21 @@ int case_rows(const jst_batch *b, const HostCase &hc)
     return b->variant < MIX_NVARIANTS ? jst::MIX_SEGMENT_ROWS[b->variant] : own_segment_rows(hc.ni, hc.nj);
 }
 
-// Work items of one case for the row march: 58-column groups x row segments of `rows` rows.
+// Work items of one case for the row march: row segments of `rows` rows x 58-column groups,
+// the column groups of a segment consecutive.  Four consecutive items share a workgroup and
+// march the same rows at the same time, so the three halo columns each wave reads on either
+// side are its neighbours' own columns, loaded at the same moment (L2 hits): against column
+// groups outermost, 2-3 % less time per inner and stage-5 launch of the 64-case batch.
 void case_items(const HostCase &hc, int rows, std::vector<TileDesc> &out)
 {
-    for (int j0 = 0; j0 < hc.nj; j0 += MIX_COLS)
-        for (int i0 = 0; i0 < hc.ni; i0 += rows)
+    for (int i0 = 0; i0 < hc.ni; i0 += rows)
+        for (int j0 = 0; j0 < hc.nj; j0 += MIX_COLS)
             out.push_back(TileDesc{hc.slot, i0, j0, std::min(i0 + rows, hc.ni)});
 }
 
 // Rebuild the work-item list and the step-end slot list.  XCD-aware: workgroup w (four
 // consecutive items) runs on XCD w % 8, so runs of consecutive items of a case - neighbouring
-// row segments, which share halo rows - are dealt to one XCD.
+// column groups and row segments, which share halo columns and rows - are dealt to one XCD.
 jst_status rebuild_lists(jst_batch *b)
 {
     std::vector<TileDesc> nat;

@@ -939,13 +939,14 @@ __global__ void shadow_kernel(const CaseMix *__restrict__ cases, int ci)
 // ---------------------------------------------------------------------------------------
 static __device__ __align__(256) TileDesc scratch[JST_NOALLOC_MAX_ITEMS];
 
-// item k -> (case slot, row segment k % nseg of `rows` rows, column group k / nseg)
-__global__ void make_items_kernel(TileDesc *__restrict__ items, int slot, int ni, int rows, int nseg,
+// item k -> (case slot, column group k % ngroups, row segment k / ngroups of `rows` rows): the
+// order of the batch's host-built lists
+__global__ void make_items_kernel(TileDesc *__restrict__ items, int slot, int ni, int rows, int ngroups,
                                   int nitems)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nitems) return;
-    const int seg = k % nseg, grp = k / nseg;
+    const int grp = k % ngroups, seg = k / ngroups;
     const int i0 = seg * rows;
     items[k] = TileDesc{slot, i0, grp * MIX_COLS, min(i0 + rows, ni)};
 }
@@ -989,7 +990,7 @@ hipError_t launch_make_items(TileDesc *items, int slot, int ni, int nj, int rows
     if (n > JST_NOALLOC_MAX_ITEMS) return hipErrorInvalidValue;
     *nitems = (int)n;
     hipLaunchKernelGGL(make_items_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, items,
-                       slot, ni, rows, nseg, (int)n);
+                       slot, ni, rows, ngroups, (int)n);
     return hipGetLastError();
 }
 
