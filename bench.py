@@ -49,9 +49,6 @@ HBM_PEAK_GBS = 8000.0
 #   mixed stage 5: dU^(4) 20 + U^n 40 + U^{n+1} 40 + dU^{n+1} 20 + geometry 28 + V/dt 4
 #                  + next dt and V/dt 8 = 160
 BYTES = {"fp64": 184.0, "mixed": 152.0}
-# row-segment length of each stage-kernel tiling variant (jst_batch_set_tiling); variant
-# 100 + L is L rows for every case
-SEGMENT_ROWS = [16, 24, 32, 40, 48, 64, 8, 4, 2, 1, "per-case"]
 DESIGN_BYTES = {"mixed_inner": 92.0, "mixed_last": 160.0}
 METRIC = "cell-stage updates/s FP32-batched vs FP64-plain (ΔCL,ΔCD≤1e-4); p99 solve latency"
 
@@ -193,11 +190,9 @@ def bench_c4(args, jst):
     # throughput: every case resident, autotuned tiles, HIP-graph replayed steps
     def resident_batch():
         bb = jst.Batch(hbm_bytes=8 << 30, max_cases=len(cases))
-        for w in cases:
-            bb.add(w["config"], w["mesh"], 10**9)
-        return bb
+        return bb, [bb.add(w["config"], w["mesh"], 10**9) for w in cases]
 
-    b = resident_batch()
+    b, ids = resident_batch()
     if args.tiling:
         variant, grid = (int(x) for x in args.tiling.split(","))
     else:
@@ -205,9 +200,11 @@ def bench_c4(args, jst):
         # from the freestream so that every case stays well inside its stable window
         variant, grid = b.tune(6)
         b.close()
-        b = resident_batch()
+        b, ids = resident_batch()
     b.set_tiling(variant, grid)
-    rows = variant - 100 if variant >= 100 else SEGMENT_ROWS[variant]
+    # the segment length of the chosen variant (variant 10: each case its own)
+    lens = {b.segment_rows(cid) for cid in ids}
+    rows = lens.pop() if len(lens) == 1 else "per-case"
     if args.warmup > 0:
         b.time_steps(args.warmup)
     # untimed: stops off, launch lists and the step graphs built and launched once; the timed

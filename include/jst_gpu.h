@@ -235,11 +235,17 @@ jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, doub
 /* Work decomposition of the stage kernel: variant 0..9 = row-segment length (16, 24, 32, 40,
  * 48, 64, 8, 4, 2 or 1 rows per wave work item of 58 columns) for every case, variant 10 =
  * each case its own length from its size alone (the shortest whose work items number at
- * most 2048, the waves the GPU keeps resident), variant 100 + L (1 <= L <= 256)
- * = L rows for every case; grid = workgroups of 4 waves (0 = one per 4 work items).  The
- * grid never changes results; the segment length changes them only at FP32 rounding level
- * (where a segment's edge rows are evaluated). */
+ * most 2048, the waves the GPU keeps resident), variant 11 = one length for every case,
+ * fitted to the whole batch whenever its cases change (the one minimising
+ * ceil(items / 2048) x (L + 5), the row steps of the busiest resident wave), variant
+ * 100 + L (1 <= L <= 256) = L rows for every case; grid = workgroups of 4 waves (0 = one
+ * per 4 work items).  The grid never changes results; the segment length changes them only
+ * at FP32 rounding level (where a segment's edge rows are evaluated), so under variant 11 a
+ * case's result depends on which other cases share the batch at that level. */
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid);
+/* Row-segment length the stage kernel runs case case_id with under the current tiling
+ * (under variant 11, the batch's fitted length for its current cases). */
+jst_status jst_batch_segment_rows(const jst_batch *b, int case_id, int *rows);
 /* Autotuner: times every variant with three grid sizes (0, 512, 1024) on the current batch
  * for trial_steps steps each, in two rounds (each candidate keeps its faster time; the steps
  * are real work and advance the cases), and keeps the fastest; returns the choice. */
@@ -288,9 +294,10 @@ typedef struct {
     int segment_rows;         /* row-segment length of the stage kernel.  0 = adaptive: each
                                  case its own length from its size alone (variant 10 of
                                  jst_batch_set_tiling: short segments, for latency) while
-                                 the GPU has waves to spare, 24 rows for every case while it
-                                 is saturated (fewer halo rows, for throughput); a switch
-                                 moves results at FP32 rounding level.  -1 = each case its
+                                 the GPU has waves to spare, one length fitted to the batch
+                                 while it is saturated (variant 11: fewer halo rows, every
+                                 item in one round where possible, for throughput); a switch
+                                 or a refit moves results at FP32 rounding level.  -1 = each case its
                                  own length always (results bitwise independent of the
                                  load).  1 .. 256 = that length for every case */
 } jst_serve_opts;

@@ -203,6 +203,7 @@ struct jst_batch {
     int *d_slots = nullptr; // slots of the batch's cases (one step-end block each)
     int nslots = 0;
     int variant = 0, grid = 0;
+    int fit_rows = 24; // segment length under MIX_VARIANT_FIT, refitted by rebuild_lists
     bool dirty = true;
     hipGraph_t graph = nullptr;      // one step
     hipGraphExec_t gexec = nullptr;
@@ -268,13 +269,40 @@ int own_segment_rows(int ni, int nj)
     return MIX_MAX_ROWS;
 }
 
-// Row-segment length of a case under the batch's tiling: the variant's length, or under
-// MIX_VARIANT_PER_CASE the case's own, which depends on nothing else in the batch.
+// One segment length for the whole batch: the one that minimises the estimated launch time,
+// in row steps of the busiest wave, ceil(items / 2048) x (L + 5) (the items are dealt round-
+// robin over the 2048 resident waves and an item of L rows walks L + 5 rows); ties go to the
+// shorter length.  This is the shortest length whose items all fit in one round when such a
+// length exists: the 64-case NACA batch gets 37 rows (2019 items), against 526 / 540 / 545 us
+// per step at 37 / 38 / 40 rows and 700 us at 36 rows (2172 items, a second round).
+int fit_segment_rows(const jst_batch *b, double *row_steps = nullptr)
+{
+    int best = 24;
+    double best_t = 1e300;
+    for (int rows = 1; rows <= MIX_MAX_ROWS; ++rows) {
+        long items = 0;
+        for (auto &hc : b->cases)
+            if (hc) items += (long)((hc->nj + MIX_COLS - 1) / MIX_COLS) * ((hc->ni + rows - 1) / rows);
+        if (items == 0) break;
+        const double t = (double)((items + 2047) / 2048) * (rows + 5);
+        if (t < best_t) {
+            best_t = t;
+            best = rows;
+        }
+    }
+    if (row_steps) *row_steps = best_t;
+    return best;
+}
+
+// Row-segment length of a case under the batch's tiling: the variant's length, under
+// MIX_VARIANT_PER_CASE the case's own, which depends on nothing else in the batch, or under
+// MIX_VARIANT_FIT the batch's fitted length as of the last list rebuild.
 int case_rows(const jst_batch *b, const HostCase &hc);
 
 int case_rows(const jst_batch *b, const HostCase &hc)
 {
     if (b->variant >= MIX_VARIANT_ROWS) return b->variant - MIX_VARIANT_ROWS;
+    if (b->variant == MIX_VARIANT_FIT) return b->fit_rows;
     return b->variant < MIX_NVARIANTS ? jst::MIX_SEGMENT_ROWS[b->variant] : own_segment_rows(hc.ni, hc.nj);
 }
 
@@ -297,6 +325,7 @@ jst_status rebuild_lists(jst_batch *b)
 {
     std::vector<TileDesc> nat;
     std::vector<int> slots;
+    if (b->variant == MIX_VARIANT_FIT) b->fit_rows = fit_segment_rows(b);
     for (auto &hc : b->cases) {
         if (!hc) continue;
         const int rows = case_rows(b, *hc);
@@ -989,10 +1018,21 @@ jst_status jst_batch_get_field(jst_batch *b, int case_id, const char *name, doub
     return JST_OK;
 }
 
+jst_status jst_batch_segment_rows(const jst_batch *b, int case_id, int *rows)
+{
+    if (!rows) return fail(JST_ERR_ARG, "bad arguments");
+    const HostCase *hc = get_case(b, case_id);
+    if (!hc) return fail(JST_ERR_ARG, "no such case");
+    // under the fitted variant the lists may not have been rebuilt since the last change of
+    // the batch: report the length the next step will use
+    *rows = b->variant == MIX_VARIANT_FIT ? fit_segment_rows(b) : case_rows(b, *hc);
+    return JST_OK;
+}
+
 jst_status jst_batch_set_tiling(jst_batch *b, int variant, int grid)
 {
     const bool rows_ok = variant > MIX_VARIANT_ROWS && variant <= MIX_VARIANT_ROWS + MIX_MAX_ROWS;
-    if (!b || variant < 0 || (variant > MIX_VARIANT_PER_CASE && !rows_ok) || grid < 0)
+    if (!b || variant < 0 || (variant > MIX_VARIANT_FIT && !rows_ok) || grid < 0)
         return fail(JST_ERR_ARG, "bad tiling");
     HIPCHK(hipStreamSynchronize(b->stream));
     if (variant != b->variant) b->dirty = true;
@@ -1152,7 +1192,7 @@ static jst_status tune_impl(jst_batch *b, int trial_steps, bool variants, int *v
 {
     if (!b || trial_steps <= 0) return fail(JST_ERR_ARG, "bad arguments");
     int best_v = b->variant, best_g = b->grid;
-    const int v0 = variants ? 0 : b->variant, v1 = variants ? MIX_VARIANT_PER_CASE + 1 : b->variant + 1;
+    const int v0 = variants ? 0 : b->variant, v1 = variants ? MIX_VARIANT_FIT + 1 : b->variant + 1;
     // grid 0: one workgroup per four work items; otherwise a persistent grid of k workgroups
     // per CU.  Each candidate is timed in two rounds and keeps its faster time, so that a
     // one-off stall in one timing does not decide the choice.
@@ -1200,8 +1240,8 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
     if (st != JST_OK) return st;
     std::unique_ptr<jst_batch> guard(b);
     // 0: every case its own segment length (from its size alone) while the GPU has waves to
-    // spare, 24-row segments for every case while it is saturated; < 0: every case its own
-    // length always (results bitwise independent of the load); else one fixed length
+    // spare, one length fitted to the whole batch while it is saturated; < 0: every case its
+    // own length always (results bitwise independent of the load); else one fixed length
     const bool adaptive = opts->segment_rows == 0;
     {
         int v = opts->segment_rows <= 0 ? MIX_VARIANT_PER_CASE : -1;
@@ -1262,23 +1302,22 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
             // Step time of each tiling, in row steps of one wave: a work item of L rows walks
             // L + 5 rows, and the launch lasts about the longer of one item and the total
             // over the 2048 waves two per SIMD keep resident.  Short own lengths win while the
-            // GPU has waves to spare; 24-row segments (fewer halo rows) once it is saturated.
-            // The tiling changes only when the other one is estimated 20 % faster.
-            double lat_total = 0.0, thr_total = 0.0, lat_item = 0.0;
+            // GPU has waves to spare; one length fitted to the batch (fewer halo rows, every
+            // item in one round where possible) once it is saturated.  The tiling changes only
+            // when the other one is estimated 20 % faster.
+            double lat_total = 0.0, lat_item = 0.0, t_fit = 0.0;
             for (auto &hc : b->cases)
                 if (hc) {
                     const long groups = (hc->nj + MIX_COLS - 1) / MIX_COLS;
                     const int rows = own_segment_rows(hc->ni, hc->nj);
                     lat_total += (double)groups * (double)((hc->ni + rows - 1) / rows) * (rows + 5);
                     lat_item = std::max(lat_item, (double)std::min(rows, hc->ni) + 5.0);
-                    thr_total += (double)groups * ((double)hc->ni + 5.0 * ((hc->ni + 23) / 24));
                 }
+            (void)fit_segment_rows(b, &t_fit);
             const double t_lat = std::max(lat_item, lat_total / 2048.0);
-            const double t_thr = std::max(29.0, thr_total / 2048.0);
-            const int v24 = 1; // MIX_SEGMENT_ROWS[1] == 24
             int want = b->variant;
-            if (b->variant == MIX_VARIANT_PER_CASE && t_thr < 0.8 * t_lat) want = v24;
-            if (b->variant == v24 && t_lat < 0.8 * t_thr) want = MIX_VARIANT_PER_CASE;
+            if (b->variant == MIX_VARIANT_PER_CASE && t_fit < 0.8 * t_lat) want = MIX_VARIANT_FIT;
+            if (b->variant == MIX_VARIANT_FIT && t_lat < 0.8 * t_fit) want = MIX_VARIANT_PER_CASE;
             if (want != b->variant) {
                 if ((st = jst_batch_set_tiling(b, want, b->grid)) != JST_OK) return st;
                 ++switches;

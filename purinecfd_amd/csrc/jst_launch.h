@@ -37,6 +37,8 @@ hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, dou
 // general row body), never on the launch grid.
 #define MIX_NVARIANTS 10
 #define MIX_VARIANT_PER_CASE MIX_NVARIANTS
+// one length for every case, fitted to the whole batch (jst_batch.cpp fit_segment_rows)
+#define MIX_VARIANT_FIT (MIX_NVARIANTS + 1)
 // variant MIX_VARIANT_ROWS + L (1 <= L <= MIX_MAX_ROWS): every case L rows per work item
 #define MIX_VARIANT_ROWS 100
 #define MIX_MAX_ROWS 256

@@ -516,6 +516,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             // keep the prefetch and U^n loads ahead of the row's arithmetic (the scheduler
             // would otherwise sink some of them next to their use and wait on all loads there)
             __builtin_amdgcn_sched_barrier(0);
+            // The row's arithmetic runs at raised wave priority: a wave that has its loads
+            // issued and its operands arriving keeps the SIMD while the other resident wave
+            // waits on memory, which spreads the two waves' load bursts apart in time
+            // (64-case batch, 40-row segments: 567 -> 545 us per step).
+            __builtin_amdgcn_s_setprio(1);
 
             // ---- row r: state and primitives, ghosts synthesised ----
             float dp, du, dv, dT, nut;
@@ -727,6 +732,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     }
                 }
             }
+
+            __builtin_amdgcn_s_setprio(0);
 
             // ---- shift the windows ----
 #pragma unroll

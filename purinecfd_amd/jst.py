@@ -37,7 +37,7 @@ EXPORTS = ["jst_last_error", "jst_config_check", "jst_mesh_parse", "jst_create",
            "jst_batch_remove", "jst_batch_step", "jst_batch_info", "jst_batch_forces", "jst_batch_set_state",
            "jst_batch_wall", "jst_batch_get_field", "jst_batch_residual_terms",
            "jst_batch_residual_terms_noalloc", "jst_batch_history",
-           "jst_batch_set_tiling", "jst_batch_tune", "jst_batch_time_begin", "jst_batch_time_end",
+           "jst_batch_set_tiling", "jst_batch_segment_rows", "jst_batch_tune", "jst_batch_time_begin", "jst_batch_time_end",
            "jst_batch_time_steps", "jst_batch_time_kernels",
            "jst_batch_counts",
            "jst_copy_probe", "jst_serve"]
@@ -163,6 +163,7 @@ def lib():
         L.jst_batch_history.argtypes = [vp, ip, C.c_long, C.c_long, dp]
         L.jst_batch_set_tiling.argtypes = [vp, ip, ip]
         L.jst_batch_tune.argtypes = [vp, ip, C.POINTER(ip), C.POINTER(ip)]
+        L.jst_batch_segment_rows.argtypes = [vp, ip, C.POINTER(ip)]
         L.jst_batch_time_begin.argtypes = [vp, ip]
         L.jst_batch_time_end.argtypes = [vp]
         L.jst_batch_time_steps.argtypes = [vp, ip, dp, dp]
@@ -490,6 +491,12 @@ class Batch:
 
     def set_tiling(self, variant: int, grid: int = 0) -> None:
         _check(lib().jst_batch_set_tiling(self._h, int(variant), int(grid)))
+
+    def segment_rows(self, cid: int) -> int:
+        """Row-segment length the stage kernel runs case ``cid`` with."""
+        r = C.c_int()
+        _check(lib().jst_batch_segment_rows(self._h, int(cid), C.byref(r)))
+        return r.value
 
     def tune(self, trial_steps: int = 5):
         v, g = C.c_int(), C.c_int()

@@ -136,7 +136,8 @@ def test_segment_length_and_grid(jst):
     cfg, text = _cases()[1]
     runs = {}
     for variant, grid in ((0, 0), (0, 3), (1, 0), (1, 16), (2, 0), (3, 0), (4, 0), (5, 0), (5, 1024), (6, 0), (6, 5),
-                          (7, 0), (7, 7), (8, 0), (8, 3), (9, 0), (9, 5), (10, 0), (10, 3)):
+                          (7, 0), (7, 7), (8, 0), (8, 3), (9, 0), (9, 5), (10, 0), (10, 3), (11, 0),
+                          (11, 5), (137, 0), (137, 3)):
         b = jst.Batch(1 << 30, 2)
         b.set_tiling(variant, grid)
         cid = b.add(cfg, text, 40)
@@ -150,6 +151,55 @@ def test_segment_length_and_grid(jst):
         base = runs[(0, 0)]
         for n in f:
             assert rel_l2(f[n], base[n]) <= 1e-6, (variant, n, rel_l2(f[n], base[n]))
+
+
+def test_batch_fitted_segment_length(jst):
+    """Variant 11 gives every case of the batch one segment length: the one minimising
+    ceil(items / 2048) x (L + 5), so the shortest whose work items fit the 2048 resident waves
+    in one round where one does.  Known answers: a lone 1025 x 513 case gets its own length
+    (5 rows: 18 column groups x 103 segments = 1854 items; 4 rows would give 2304); thirty 128 x 256-cell cases get 10
+    rows (5 groups x 13 segments x 30 = 1950 items; 9 rows would give 2250, a second round);
+    the 64-case NACA batch gets 37 rows (2019 items; 36 rows give 2172).  The length follows
+    the batch as cases come and go, and a refit moves results only at FP32 rounding."""
+    from purinecfd_amd import workloads
+    b = jst.Batch(1 << 30, 1)
+    b.set_tiling(11, 0)
+    w = workloads.c3_naca_large()
+    cid = b.add(w["config"], w["mesh"], 10)
+    assert b.segment_rows(cid) == 5
+    b.set_tiling(10, 0)
+    assert b.segment_rows(cid) == 5
+    b.close()
+
+    cfg, _ = _cases()[1]
+    text = ellipse_mesh(129, 256, 1.04)  # 128 x 256 cells
+    b = jst.Batch(4 << 30, 32)
+    b.set_tiling(11, 0)
+    ids = [b.add(cfg, text, 20) for _ in range(30)]
+    assert {b.segment_rows(c) for c in ids} == {10}
+    b.run(steps_per_round=10)
+    one = jst.Batch(1 << 30, 1)
+    one.set_tiling(11, 0)
+    c1 = one.add(cfg, text, 20)
+    assert one.segment_rows(c1) == 1  # alone: 5 groups x 128 one-row segments = 640 items
+    one.run(steps_per_round=10)
+    f1 = one.forces(c1)
+    one.close()
+    for c in ids:
+        f = b.forces(c)
+        assert b.info(c)["step"] == 20
+        assert abs(f["cl"] - f1["cl"]) <= 1e-5 and abs(f["cd"] - f1["cd"]) <= 1e-5
+    for c in ids[1:]:
+        b.remove(c)
+    assert b.segment_rows(ids[0]) == 1  # refitted to the one case left
+    b.close()
+
+    cases = workloads.c4_batch()
+    b = jst.Batch(8 << 30, 64)
+    b.set_tiling(11, 512)
+    ids = [b.add(w["config"], w["mesh"], 10) for w in cases]
+    assert {b.segment_rows(c) for c in ids} == {37}
+    b.close()
 
 
 def test_graph_replays_match_eager_launches(jst):
@@ -254,7 +304,7 @@ def test_tuner_choice_is_valid_and_its_steps_are_real_work(jst):
     cases = _cases()
     b = jst.Batch(1 << 30, 4)
     ids = [b.add(cfg, text, 10 ** 6) for cfg, text in cases]
-    nvar = 11  # row-segment lengths 16 24 32 40 48 64 8 4 2 1, and each case its own
+    nvar = 12  # row-segment lengths 16 24 32 40 48 64 8 4 2 1, each case its own, batch-fitted
     b.set_tiling(nvar - 1, 0)
     with pytest.raises(jst.JstError):
         b.set_tiling(nvar, 0)
