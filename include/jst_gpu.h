@@ -237,7 +237,7 @@ jst_status jst_batch_history(jst_batch *b, int case_id, long first, long n, doub
  * each case its own length from its size alone (the shortest whose work items number at
  * most 2048, the waves the GPU keeps resident), variant 11 = one length for every case,
  * fitted to the whole batch whenever its cases change (the one minimising
- * ceil(items / 2048) x (L + 5), the row steps of the busiest resident wave), variant
+ * ceil(items / 2048) x (L + 6), the row steps of the busiest resident wave), variant
  * 100 + L (1 <= L <= 256) = L rows for every case; grid = workgroups of 4 waves (0 = one
  * per 4 work items).  The grid never changes results; the segment length changes them only
  * at FP32 rounding level (where a segment's edge rows are evaluated), so under variant 11 a

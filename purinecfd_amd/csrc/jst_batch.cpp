@@ -250,10 +250,6 @@ jst_status drop_graph(jst_batch *b)
     return JST_OK;
 }
 
-// log2 of the rows per step-end partial: 4-row blocks, 2-row blocks under 2-row segments and
-// 1-row blocks under 1-row segments (a segment covers whole blocks, so that each partial has
-// one writer)
-int block_shift_for(int rows) { return rows % 4 == 0 ? 2 : (rows % 2 == 0 ? 1 : 0); }
 
 // A case's own row-segment length, from its size alone: the shortest segment whose work items
 // (58-column groups x segments) all fit in the 2048 waves the GPU keeps resident at once (two
@@ -270,10 +266,10 @@ int own_segment_rows(int ni, int nj)
 }
 
 // One segment length for the whole batch: the one that minimises the estimated launch time,
-// in row steps of the busiest wave, ceil(items / 2048) x (L + 5) (the items are dealt round-
-// robin over the 2048 resident waves and an item of L rows walks L + 5 rows); ties go to the
-// shorter length.  This is the shortest length whose items all fit in one round when such a
-// length exists: the 64-case NACA batch gets 37 rows (2019 items), against 526 / 540 / 545 us
+// in row steps of the busiest wave, ceil(items / 2048) x (L + 6) (the items are dealt round-
+// robin over the 2048 resident waves, and an item of L rows takes L + 6 row steps: 3 to fill
+// the row pipeline, 3 to drain it); ties go to the shorter length.  This is the shortest
+// length whose items all fit in one round when such a length exists: the 64-case NACA batch gets 37 rows (2019 items), against 526 / 540 / 545 us
 // per step at 37 / 38 / 40 rows and 700 us at 36 rows (2172 items, a second round).
 int fit_segment_rows(const jst_batch *b, double *row_steps = nullptr)
 {
@@ -284,7 +280,7 @@ int fit_segment_rows(const jst_batch *b, double *row_steps = nullptr)
         for (auto &hc : b->cases)
             if (hc) items += (long)((hc->nj + MIX_COLS - 1) / MIX_COLS) * ((hc->ni + rows - 1) / rows);
         if (items == 0) break;
-        const double t = (double)((items + 2047) / 2048) * (rows + 5);
+        const double t = (double)((items + 2047) / 2048) * (rows + 6);
         if (t < best_t) {
             best_t = t;
             best = rows;
@@ -331,11 +327,11 @@ jst_status rebuild_lists(jst_batch *b)
         const int rows = case_rows(b, *hc);
         case_items(*hc, rows, nat);
         slots.push_back(hc->slot);
-        // the step-end block size follows the segment length (read by stage 5 and the step end)
-        const int bs = block_shift_for(rows);
-        if (hc->dev.bshift != bs) {
-            hc->dev.bshift = bs;
-            HIPCHK(hipMemcpyAsync(&b->d_cases[hc->slot].bshift, &hc->dev.bshift, sizeof(int),
+        // stage 5 leaves one step-end partial per (segment, column group): the segment length
+        // is read by stage 5 and the step end
+        if (hc->dev.seg_rows != rows) {
+            hc->dev.seg_rows = rows;
+            HIPCHK(hipMemcpyAsync(&b->d_cases[hc->slot].seg_rows, &hc->dev.seg_rows, sizeof(int),
                                   hipMemcpyHostToDevice, b->stream));
         }
     }
@@ -706,7 +702,7 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     cm.nj = hc->nj;
     cm.pitch = hc->pitch;
     cm.npost = hc->npost;
-    cm.bshift = block_shift_for(case_rows(b, *hc));
+    cm.seg_rows = case_rows(b, *hc);
     cm.sb = F(L.sb);
     for (int q = 0; q < 2; ++q) {
         cm.u64[q] = D(L.u64[q]);
@@ -1300,7 +1296,7 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
         }
         if (adaptive) {
             // Step time of each tiling, in row steps of one wave: a work item of L rows walks
-            // L + 5 rows, and the launch lasts about the longer of one item and the total
+            // L + 6 rows, and the launch lasts about the longer of one item and the total
             // over the 2048 waves two per SIMD keep resident.  Short own lengths win while the
             // GPU has waves to spare; one length fitted to the batch (fewer halo rows, every
             // item in one round where possible) once it is saturated.  The tiling changes only
@@ -1310,8 +1306,8 @@ jst_status jst_serve(const jst_request *reqs, int n, const jst_serve_opts *opts,
                 if (hc) {
                     const long groups = (hc->nj + MIX_COLS - 1) / MIX_COLS;
                     const int rows = own_segment_rows(hc->ni, hc->nj);
-                    lat_total += (double)groups * (double)((hc->ni + rows - 1) / rows) * (rows + 5);
-                    lat_item = std::max(lat_item, (double)std::min(rows, hc->ni) + 5.0);
+                    lat_total += (double)groups * (double)((hc->ni + rows - 1) / rows) * (rows + 6);
+                    lat_item = std::max(lat_item, (double)std::min(rows, hc->ni) + 6.0);
                 }
             (void)fit_segment_rows(b, &t_fit);
             const double t_lat = std::max(lat_item, lat_total / 2048.0);

@@ -96,7 +96,7 @@ struct MixConst {
 struct CaseMix {
     int ni, nj, pitch;
     int npost;                    // dt-refresh blocks of this case (fixed 1024-cell chunks)
-    int bshift;                   // log2 of the rows per step-end block (from the segment length)
+    int seg_rows;                 // rows per row segment of the stage kernel's work items
     double *u64[2];               // U^n / U^{n+1}, 5 planes each
     float *sb;                    // FP32 base dU^n = U^n - U_inf, 5 planes
     float *s32[2];                // FP32 stage states dU^(k), 5 planes each (ping-pong)
@@ -105,8 +105,8 @@ struct CaseMix {
     const float *vol, *inv_vol, *inv_wd2;
     const double *tnx64, *tny64, *tmx64, *tmy64, *cx64, *cy64; // far-field support (FP64)
     float *ratio[2], *localdt[2]; // frozen per step; [rot] is read by the step, [rot^1] written
-    double *rpart;                // residual partial sums per (2-row block, 58-column group)
-    double *dpart;                // next-dt minima per (2-row block, 58-column group)
+    double *rpart;                // residual partial sums per (row segment, 58-column group)
+    double *dpart;                // next-dt minima per (row segment, 58-column group)
     double *res_hist;
     int res_cap;
     CaseCtl *ctl;

@@ -28,13 +28,12 @@ hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, dou
                               hipStream_t s);
 
 // FP32-mixed row-march variants: rows per work item (one wave per 58 columns x that many
-// rows), or MIX_VARIANT_PER_CASE: each case its own length from its size alone; the
-// autotuner picks one per batch.  Stage 5 leaves one residual sum and one dt minimum per
-// (block of 2^bshift rows, column group), bshift per case (CaseMix::bshift): 2, or 1 under
-// 2-row and 0 under 1-row segments (a segment covers whole blocks, so that each partial has
-// one writer).  Results
-// depend on the variant at FP32 rounding level only (where a segment's edge rows run the
-// general row body), never on the launch grid.
+// rows), MIX_VARIANT_PER_CASE: each case its own length from its size alone, or
+// MIX_VARIANT_FIT: one length fitted to the batch; the autotuner picks one per batch.
+// Stage 5 leaves one residual sum and one dt minimum per (row segment, column group), from
+// the case's segment length (CaseMix::seg_rows).  Results depend on the variant at FP32
+// rounding level only (where a segment's edge rows run the general row body; the residual's
+// FP64 summation order follows the segments), never on the launch grid.
 #define MIX_NVARIANTS 10
 #define MIX_VARIANT_PER_CASE MIX_NVARIANTS
 // one length for every case, fitted to the whole batch (jst_batch.cpp fit_segment_rows)

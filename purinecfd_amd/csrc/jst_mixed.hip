@@ -367,9 +367,9 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned q, unsign
 // Every face is evaluated once; the right n face of a lane is its neighbour's left face.
 // MIX_INNER (stages 1-4): U^(k) = U^n - a_k dt/V R in FP32 from the FP32 base dU^n.
 // MIX_LAST (stage 5): FP64 U^{n+1} = U^n - dt/V R, the new FP32 base, the next frozen dt
-// and V/dt, and the step-end partials per (2^bshift-row block, 58-column group): summed down each
-// column in row order, then across the group's columns by a fixed butterfly, so they do not
-// depend on the segment length or the batch.
+// and V/dt, and the step-end partials per (row segment, 58-column group): summed down each
+// column in row order, then across the group's columns by a fixed butterfly, so they depend
+// on the case's segment length but never on the launch grid or on the other cases.
 // MIX_TERMS: export R = Fc - Fv - Fd and S of the current state (no update).
 // ---------------------------------------------------------------------------------------
 template <int MODE>
@@ -390,7 +390,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         const unsigned plane = (unsigned)(ni * pitch);
         const int is = w.i0, J0 = w.j0, ie = w.t;
         const int ngroups = (nj + MIX_COLS - 1) / MIX_COLS;
-        const int bshift = C.bshift, bmask = (1 << bshift) - 1; // step-end block: 2^bshift rows
+        // stage 5: this item's row segment (segments start at multiples of the case's length)
+        const int seg = MODE == MIX_LAST ? is / C.seg_rows : 0;
         // control words: read once per item, made wave-uniform so that every pointer and
         // branch derived from them lives in scalar registers
         const int status = __builtin_amdgcn_readfirstlane(*(volatile int *)&ctl->status);
@@ -697,7 +698,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                         lam += 4.0f * visc_radius32(k, rho, inv, pr, (float)u1[4], aa, bb, cc, dd, VOL[0]);
                     const float ldt = k.cfl * VOL[0] / lam;
                     const double dr = u1[0] - former[0];
-                    // this column's residual sum and dt minimum over a 2^bshift-row block
+                    // this column's residual sum and dt minimum over the segment
                     racc += own ? dr * dr : 0.0;
                     dacc = own ? fminf(dacc, ldt) : dacc;
                     if (own) {
@@ -712,9 +713,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
                                                          (unsigned long long)((long long)c * nj + jg));
                     }
-                    if ((~c & bmask) == 0 || c == ie - 1) { // block end (segments start at block starts)
+                    if (c == ie - 1) { // segment end
                         // sum and minimum over the wave's columns: a fixed butterfly (every lane
-                        // ends with the same value), then one partial per (block, column group)
+                        // ends with the same value), then one partial per (segment, column group)
                         double rs = racc;
                         float dm = dacc;
 #pragma unroll
@@ -723,7 +724,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                             dm = fminf(dm, __shfl_xor(dm, off, 64));
                         }
                         if (lane == 0) {
-                            const unsigned ch = (unsigned)((c >> bshift) * ngroups + J0 / MIX_COLS);
+                            const unsigned ch = (unsigned)(seg * ngroups + J0 / MIX_COLS);
                             stg(C.rpart, ch, rs);
                             stg(C.dpart, ch, (double)dm);
                         }
@@ -794,7 +795,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
 
 // ---------------------------------------------------------------------------------------
 // End of step, one block per case: residual sqrt(sum (rho^{n+1} - rho^n)^2 / NI / NJ)
-// (jst-rs solver.rs:190-200) and the next global dt from the per-(2^bshift-row block, column group)
+// (jst-rs solver.rs:190-200) and the next global dt from the per-(row segment, column group)
 // partials in a fixed order, totaltime += dt, step += 1, buffer rotation, and the stop decision
 // (convergence / relative drop / step limit, or the non-physical record).
 // ---------------------------------------------------------------------------------------
@@ -808,8 +809,7 @@ __global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__
     CaseCtl *ctl = C.ctl;
     const int tid = threadIdx.x;
     if (*(volatile int *)&ctl->status != JST_RUN_ACTIVE) return;
-    const int bshift = C.bshift;
-    const int n = ((C.ni + (1 << bshift) - 1) >> bshift) * ((C.nj + MIX_COLS - 1) / MIX_COLS);
+    const int n = ((C.ni + C.seg_rows - 1) / C.seg_rows) * ((C.nj + MIX_COLS - 1) / MIX_COLS);
     double s = 0.0, m = __builtin_huge_val();
     // sequential per lane (fixed order); unrolled so that the loads are in flight together
 #pragma unroll 16

@@ -155,7 +155,7 @@ def test_segment_length_and_grid(jst):
 
 def test_batch_fitted_segment_length(jst):
     """Variant 11 gives every case of the batch one segment length: the one minimising
-    ceil(items / 2048) x (L + 5), so the shortest whose work items fit the 2048 resident waves
+    ceil(items / 2048) x (L + 6), so the shortest whose work items fit the 2048 resident waves
     in one round where one does.  Known answers: a lone 1025 x 513 case gets its own length
     (5 rows: 18 column groups x 103 segments = 1854 items; 4 rows would give 2304); thirty 128 x 256-cell cases get 10
     rows (5 groups x 13 segments x 30 = 1950 items; 9 rows would give 2250, a second round);
