@@ -79,7 +79,7 @@ struct Arena {
 
 // Byte layout of one case inside its arena block.
 struct CaseLayout {
-    size_t u64[2], sb, s32[2], g64[14], g32[7], ratio[2], localdt[2], rpart, dpart, res_hist, posts, scratch, dbg;
+    size_t u64[2], sb, s32[2], g64[14], g32[6], ratio[2], localdt[2], rpart, dpart, res_hist, posts, scratch, dbg;
     size_t total;
 };
 
@@ -100,11 +100,11 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     for (int q = 0; q < 6; ++q) L.g64[q] = carve(plane * 8);
     for (int q = 6; q < 10; ++q) L.g64[q] = carve(fplane * 8);
     for (int q = 10; q < 14; ++q) L.g64[q] = carve(plane * 8);
-    // FP32 geometry: tnx tny (fplane) | nnx nny vol inv_vol inv_wd2 (plane), then V/dt and
+    // FP32 geometry: tnx tny (fplane) | nnx nny vol inv_wd2 (plane), then V/dt and
     // the local dt.  The stage kernels reach all of them through one buffer descriptor that
     // starts at tnx and ends with localdt[1]: keep tnx first and localdt[1] last.
     for (int q = 0; q < 2; ++q) L.g32[q] = carve(fplane * 4);
-    for (int q = 2; q < 7; ++q) L.g32[q] = carve(plane * 4);
+    for (int q = 2; q < 6; ++q) L.g32[q] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
     const size_t nparts = (size_t)ni * ((nj + MIX_COLS - 1) / MIX_COLS); // 1-row blocks at most
@@ -712,15 +712,14 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     double **g64p[14] = {&g.vol, &g.cx, &g.cy, &g.wd, &g.inv_vol, &g.inv_wd2, &g.tnx, &g.tny,
                          &g.tmx, &g.tmy, &g.nnx, &g.nny, &g.nmx, &g.nmy};
     for (int q = 0; q < 14; ++q) *g64p[q] = D(L.g64[q]);
-    float *g32[7];
-    for (int q = 0; q < 7; ++q) g32[q] = F(L.g32[q]);
+    float *g32[6];
+    for (int q = 0; q < 6; ++q) g32[q] = F(L.g32[q]);
     cm.tnx = g32[0];
     cm.tny = g32[1];
     cm.nnx = g32[2];
     cm.nny = g32[3];
     cm.vol = g32[4];
-    cm.inv_vol = g32[5];
-    cm.inv_wd2 = g32[6];
+    cm.inv_wd2 = g32[5];
     cm.tnx64 = g.tnx;
     cm.tny64 = g.tny;
     cm.tmx64 = g.tmx;
@@ -756,8 +755,7 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     HIPCHK(jst::launch_to_f32(g.nnx, g32[2], hc->plane, s));
     HIPCHK(jst::launch_to_f32(g.nny, g32[3], hc->plane, s));
     HIPCHK(jst::launch_to_f32(g.vol, g32[4], hc->plane, s));
-    HIPCHK(jst::launch_to_f32(g.inv_vol, g32[5], hc->plane, s));
-    HIPCHK(jst::launch_to_f32(g.inv_wd2, g32[6], hc->plane, s));
+    HIPCHK(jst::launch_to_f32(g.inv_wd2, g32[5], hc->plane, s));
     HIPCHK(hipMemcpyAsync(hc->d_posts, hc->posts.data(), hc->posts.size() * sizeof(PostDesc),
                           hipMemcpyHostToDevice, s));
     // freestream start (jst-rs state.rs:174-190), packed

@@ -102,7 +102,7 @@ struct CaseMix {
     float *s32[2];                // FP32 stage states dU^(k), 5 planes each (ping-pong)
     const float *tnx, *tny;       // FP32 tau normals (ni+1) x pitch
     const float *nnx, *nny;       // FP32 n normals
-    const float *vol, *inv_vol, *inv_wd2;
+    const float *vol, *inv_wd2;   // (1/V is formed from vol in the kernel: 4 B/cell less to read)
     const double *tnx64, *tny64, *tmx64, *tmy64, *cx64, *cy64; // far-field support (FP64)
     float *ratio[2], *localdt[2]; // frozen per step; [rot] is read by the step, [rot^1] written
     double *rpart;                // residual partial sums per (row segment, 58-column group)

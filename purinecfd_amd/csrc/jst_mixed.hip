@@ -408,13 +408,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         const float *src = (MODE == MIX_TERMS || stage == 0) ? C.sb : C.s32[(stage - 1) & 1];
         float *sdst = C.s32[stage & 1];
         // FP32 geometry, frozen V/dt and local dt: the case block holds them in the order tnx,
-        // tny, nnx, nny, vol, inv_vol, inv_wd2, ratio[0..1], localdt[0..1], so one descriptor
-        // from tnx with a byte offset per array covers them
+        // tny, nnx, nny, vol, inv_wd2, ratio[0..1], localdt[0..1], so one descriptor from tnx
+        // with a byte offset per array covers them
         const char *g0 = (const char *)C.tnx;
         auto goff = [&](const float *p) { return (unsigned)((const char *)p - g0); };
         const __amdgpu_buffer_rsrc_t r_g = planes_rsrc(g0, goff(C.localdt[1]) + plane * 4u);
         const unsigned o_tny = goff(C.tny), o_nnx = goff(C.nnx), o_nny = goff(C.nny),
-                       o_vol = goff(C.vol), o_iv = goff(C.inv_vol), o_iw = goff(C.inv_wd2),
+                       o_vol = goff(C.vol), o_iw = goff(C.inv_wd2),
                        o_rt = goff(C.ratio[rot]), o_rt_next = goff(C.ratio[rot ^ 1]),
                        o_ldt = goff(C.localdt[rot]), o_ldt_next = goff(C.localdt[rot ^ 1]);
         const double a_dt = alpha * ctl->dt;
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         double racc = 0.0;       // stage 5: sum of (rho^{n+1} - rho^n)^2 over this row block
         float dacc = __builtin_huge_valf(); // stage 5: minimum next local dt over the block
         // bundle of row r, loaded during step r-1
-        float bu[5] = {}, brt = 0.f, btn[2] = {}, bnn[2] = {}, biv = 0.f, bvol = 0.f, biw = 0.f;
+        float bu[5] = {}, brt = 0.f, btn[2] = {}, bnn[2] = {}, bvol = 0.f, biw = 0.f;
         float bsb[5] = {}, bldt = 0.f;
         double bfm[5] = {}; // stage 5: U^n of the row the step updates
 
@@ -478,7 +478,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                 const unsigned q = cidx(x, jw, pitch);
                 bnn[0] = bld(r_g, q, o_nnx);
                 bnn[1] = bld(r_g, q, o_nny);
-                biv = bld(r_g, q, o_iv);
                 bvol = bld(r_g, q, o_vol);
                 biw = bld(r_g, q, o_iw);
             }
@@ -503,7 +502,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             constexpr bool ST = decltype(steady)::value;
             // ---- take the bundle of row r, prefetch row r+1 ----
             float un[5], rt_new = brt, tn_new[2] = {btn[0], btn[1]}, nn_new[2] = {bnn[0], bnn[1]};
-            const float iv_new = biv, vol_new = bvol, iw_new = biw;
+            // 1/V from the FP32 volume: one reciprocal per row instead of 4 B/cell more to
+            // read (the loads bound this kernel: 64-case batch 520 -> 505 us per step)
+            const float iv_new = frcp(bvol), vol_new = bvol, iw_new = biw;
             float base[5];
             double former[5];
             float ldt_c = bldt;

@@ -146,7 +146,7 @@ def test_uneven_batch_of_64_cases(jst, oracle_lib):
 
 def test_serving_stream_matches_alone_runs_and_the_oracle(jst, oracle_lib):
     """A seeded 2 s Poisson stream of the serving mix (cylinders 40x17 .. 160x65 to a
-    6-decade drop, draws of the uneven NACA distribution capped at 800 steps), served with
+    6-decade drop, draws of the uneven NACA distribution capped at 500 steps), served with
     each case's own segment length (segment_rows -1): every request ends bitwise where the same
     case run alone ends (steps, status, residual, CL, CD); one converged cylinder of each size
     is within 1e-4 of the oracle in CL and CD."""
