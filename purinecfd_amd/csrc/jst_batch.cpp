@@ -79,7 +79,7 @@ struct Arena {
 
 // Byte layout of one case inside its arena block.
 struct CaseLayout {
-    size_t u64[2], sb, s32[2], g64[14], g32[6], ratio[2], localdt[2], rpart, dpart, res_hist, posts, scratch, dbg;
+    size_t u64[2], sb, s32[2], g64[14], tau2, cell4, ratio[2], localdt[2], rpart, dpart, res_hist, posts, scratch, dbg;
     size_t total;
 };
 
@@ -100,11 +100,12 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     for (int q = 0; q < 6; ++q) L.g64[q] = carve(plane * 8);
     for (int q = 6; q < 10; ++q) L.g64[q] = carve(fplane * 8);
     for (int q = 10; q < 14; ++q) L.g64[q] = carve(plane * 8);
-    // FP32 geometry: tnx tny (fplane) | nnx nny vol inv_wd2 (plane), then V/dt and
-    // the local dt.  The stage kernels reach all of them through one buffer descriptor that
-    // starts at tnx and ends with localdt[1]: keep tnx first and localdt[1] last.
-    for (int q = 0; q < 2; ++q) L.g32[q] = carve(fplane * 4);
-    for (int q = 2; q < 6; ++q) L.g32[q] = carve(plane * 4);
+    // FP32 geometry: tau normals (nx, ny) per face (fplane pairs), then (n-normal nx, ny, V,
+    // 1/d^2) per cell (plane quads), then V/dt and the local dt.  The stage kernels reach all
+    // of them through one buffer descriptor that starts at tau2 and ends with localdt[1]:
+    // keep tau2 first and localdt[1] last.
+    L.tau2 = carve(fplane * 8);
+    L.cell4 = carve(plane * 16);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
     const size_t nparts = (size_t)ni * ((nj + MIX_COLS - 1) / MIX_COLS); // 1-row blocks at most
@@ -712,14 +713,9 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     double **g64p[14] = {&g.vol, &g.cx, &g.cy, &g.wd, &g.inv_vol, &g.inv_wd2, &g.tnx, &g.tny,
                          &g.tmx, &g.tmy, &g.nnx, &g.nny, &g.nmx, &g.nmy};
     for (int q = 0; q < 14; ++q) *g64p[q] = D(L.g64[q]);
-    float *g32[6];
-    for (int q = 0; q < 6; ++q) g32[q] = F(L.g32[q]);
-    cm.tnx = g32[0];
-    cm.tny = g32[1];
-    cm.nnx = g32[2];
-    cm.nny = g32[3];
-    cm.vol = g32[4];
-    cm.inv_wd2 = g32[5];
+    float *tau2 = F(L.tau2), *cell4 = F(L.cell4);
+    cm.tau2 = tau2;
+    cm.cell4 = cell4;
     cm.tnx64 = g.tnx;
     cm.tny64 = g.tny;
     cm.tmx64 = g.tmx;
@@ -750,12 +746,12 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
     double *d_xy = D(L.total);
     HIPCHK(hipMemcpyAsync(d_xy, mesh.xy.data(), mesh.xy.size() * 8, hipMemcpyHostToDevice, s));
     HIPCHK(jst::launch_geometry(d_xy, mesh.n_theta, hc->ni, hc->nj, hc->pitch, g, s));
-    HIPCHK(jst::launch_to_f32(g.tnx, g32[0], hc->fplane, s));
-    HIPCHK(jst::launch_to_f32(g.tny, g32[1], hc->fplane, s));
-    HIPCHK(jst::launch_to_f32(g.nnx, g32[2], hc->plane, s));
-    HIPCHK(jst::launch_to_f32(g.nny, g32[3], hc->plane, s));
-    HIPCHK(jst::launch_to_f32(g.vol, g32[4], hc->plane, s));
-    HIPCHK(jst::launch_to_f32(g.inv_wd2, g32[5], hc->plane, s));
+    HIPCHK(jst::launch_to_f32_strided(g.tnx, tau2, hc->fplane, 2, 0, s));
+    HIPCHK(jst::launch_to_f32_strided(g.tny, tau2, hc->fplane, 2, 1, s));
+    HIPCHK(jst::launch_to_f32_strided(g.nnx, cell4, hc->plane, 4, 0, s));
+    HIPCHK(jst::launch_to_f32_strided(g.nny, cell4, hc->plane, 4, 1, s));
+    HIPCHK(jst::launch_to_f32_strided(g.vol, cell4, hc->plane, 4, 2, s));
+    HIPCHK(jst::launch_to_f32_strided(g.inv_wd2, cell4, hc->plane, 4, 3, s));
     HIPCHK(hipMemcpyAsync(hc->d_posts, hc->posts.data(), hc->posts.size() * sizeof(PostDesc),
                           hipMemcpyHostToDevice, s));
     // freestream start (jst-rs state.rs:174-190), packed

@@ -100,9 +100,10 @@ struct CaseMix {
     double *u64[2];               // U^n / U^{n+1}, 5 planes each
     float *sb;                    // FP32 base dU^n = U^n - U_inf, 5 planes
     float *s32[2];                // FP32 stage states dU^(k), 5 planes each (ping-pong)
-    const float *tnx, *tny;       // FP32 tau normals (ni+1) x pitch
-    const float *nnx, *nny;       // FP32 n normals
-    const float *vol, *inv_wd2;   // (1/V is formed from vol in the kernel: 4 B/cell less to read)
+    // FP32 geometry, interleaved so that the row march loads it with two wide loads per row:
+    // tau2[2f + 0/1] = tau-normal (nx, ny) of face f, (ni+1) x pitch faces; cell4[4c + 0..3] =
+    // n-normal (nx, ny), V and 1/d^2 of cell c, ni x pitch cells (1/V is formed from V)
+    const float *tau2, *cell4;
     const double *tnx64, *tny64, *tmx64, *tmy64, *cx64, *cy64; // far-field support (FP64)
     float *ratio[2], *localdt[2]; // frozen per step; [rot] is read by the step, [rot^1] written
     double *rpart;                // residual partial sums per (row segment, 58-column group)

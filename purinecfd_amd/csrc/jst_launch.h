@@ -48,6 +48,8 @@ hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int n
 hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s);
 hipError_t launch_dt_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, hipStream_t s);
 hipError_t launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);
+// b[i * stride + lane] = (float)a[i], i < n: FP32 copies interleaved into a packed array
+hipError_t launch_to_f32_strided(const double *a, float *b, size_t n, int stride, int lane, hipStream_t s);
 
 // Allocation-free work-item lists: a fixed device array of JST_NOALLOC_MAX_ITEMS work items
 // per device.  noalloc_scratch returns its address on `device` (looked up once per device

@@ -342,6 +342,17 @@ __device__ __forceinline__ double bld(__amdgpu_buffer_rsrc_t r, unsigned q, unsi
 {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, q * 8u, soff, 0));
 }
+// Element q of an interleaved array of float pairs / quads (one wide load per lane).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f32x2 bld2(__amdgpu_buffer_rsrc_t r, unsigned q, unsigned soff)
+{
+    return __builtin_bit_cast(f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, q * 8u, soff, 0));
+}
+__device__ __forceinline__ f32x4 bld4(__amdgpu_buffer_rsrc_t r, unsigned q, unsigned soff)
+{
+    return __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, q * 16u, soff, 0));
+}
 __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned q, unsigned soff, float v)
 {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, q * 4u, soff, 0);
@@ -407,14 +418,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         }
         const float *src = (MODE == MIX_TERMS || stage == 0) ? C.sb : C.s32[(stage - 1) & 1];
         float *sdst = C.s32[stage & 1];
-        // FP32 geometry, frozen V/dt and local dt: the case block holds them in the order tnx,
-        // tny, nnx, nny, vol, inv_wd2, ratio[0..1], localdt[0..1], so one descriptor from tnx
-        // with a byte offset per array covers them
-        const char *g0 = (const char *)C.tnx;
+        // FP32 geometry, frozen V/dt and local dt: the case block holds them in the order tau2
+        // (tau normal pairs), cell4 (n normal, V, 1/d^2 quads), ratio[0..1], localdt[0..1], so
+        // one descriptor from tau2 with a byte offset per array covers them
+        const char *g0 = (const char *)C.tau2;
         auto goff = [&](const float *p) { return (unsigned)((const char *)p - g0); };
         const __amdgpu_buffer_rsrc_t r_g = planes_rsrc(g0, goff(C.localdt[1]) + plane * 4u);
-        const unsigned o_tny = goff(C.tny), o_nnx = goff(C.nnx), o_nny = goff(C.nny),
-                       o_vol = goff(C.vol), o_iw = goff(C.inv_wd2),
+        const unsigned o_c4 = goff(C.cell4),
                        o_rt = goff(C.ratio[rot]), o_rt_next = goff(C.ratio[rot ^ 1]),
                        o_ldt = goff(C.localdt[rot]), o_ldt_next = goff(C.localdt[rot ^ 1]);
         const double a_dt = alpha * ctl->dt;
@@ -470,16 +480,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                 brt = bld(r_g, cidx(rc, jw, pitch), o_rt);
             }
             if (ST || (r >= 0 && r <= ni && r >= is - 1 && r <= ie + 1)) {
-                btn[0] = bld(r_g, cidx(r, jw, pitch), 0u);
-                btn[1] = bld(r_g, cidx(r, jw, pitch), o_tny);
+                const f32x2 t = bld2(r_g, cidx(r, jw, pitch), 0u);
+                btn[0] = t.x;
+                btn[1] = t.y;
             }
             const int x = r - 1;
             if (ST || (x >= 0 && x < ni && x >= is - 1 && x <= ie)) {
-                const unsigned q = cidx(x, jw, pitch);
-                bnn[0] = bld(r_g, q, o_nnx);
-                bnn[1] = bld(r_g, q, o_nny);
-                bvol = bld(r_g, q, o_vol);
-                biw = bld(r_g, q, o_iw);
+                const f32x4 g = bld4(r_g, cidx(x, jw, pitch), o_c4);
+                bnn[0] = g.x;
+                bnn[1] = g.y;
+                bvol = g.z;
+                biw = g.w;
             }
             const int c = r - 3;
             if (MODE != MIX_TERMS && (ST || (c >= is && c < ie))) {
@@ -880,13 +891,13 @@ __global__ __launch_bounds__(NT) void mix_dt_kernel(const CaseMix *__restrict__ 
         const double cs = sqrt(k.gamma * p * inv);
         const int jp = j + 1 < nj ? j + 1 : 0;
         const int cu = cidx(i + 1, j, pitch), cr = cidx(i, jp, pitch);
-        const double aa = 0.5 * ((double)C.tnx[c] + (double)C.tnx[cu]);
-        const double bb = 0.5 * ((double)C.tny[c] + (double)C.tny[cu]);
-        const double ccv = 0.5 * ((double)C.nnx[c] + (double)C.nnx[cr]);
-        const double ddv = 0.5 * ((double)C.nny[c] + (double)C.nny[cr]);
+        const double aa = 0.5 * ((double)C.tau2[2 * c] + (double)C.tau2[2 * cu]);
+        const double bb = 0.5 * ((double)C.tau2[2 * c + 1] + (double)C.tau2[2 * cu + 1]);
+        const double ccv = 0.5 * ((double)C.cell4[4 * c] + (double)C.cell4[4 * cr]);
+        const double ddv = 0.5 * ((double)C.cell4[4 * c + 1] + (double)C.cell4[4 * cr + 1]);
         const double conv = fabs(vx * aa + vy * bb) + fabs(vx * ccv + vy * ddv);
         const double acou = cs * (sqrt(aa * aa + bb * bb) + sqrt(ccv * ccv + ddv * ddv));
-        const double vol = (double)C.vol[c];
+        const double vol = (double)C.cell4[4 * c + 2];
         const double dtc = k.cfl * vol / (conv + acou);
         double ldt = dtc;
         if (k.visc_dt) {
@@ -925,6 +936,13 @@ __global__ void to_f32_kernel(const double *__restrict__ a, float *__restrict__ 
 {
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         b[q] = (float)a[q];
+}
+
+__global__ void to_f32_strided_kernel(const double *__restrict__ a, float *__restrict__ b, size_t n,
+                                      int stride, int lane)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        b[q * stride + lane] = (float)a[q];
 }
 
 __global__ void shadow_kernel(const CaseMix *__restrict__ cases, int ci)
@@ -1038,6 +1056,14 @@ hipError_t launch_to_f32(const double *a, float *b, size_t n, hipStream_t s)
 {
     int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(to_f32_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_f32_strided(const double *a, float *b, size_t n, int stride, int lane, hipStream_t s)
+{
+    int grid = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(to_f32_strided_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, s, a, b, n,
+                       stride, lane);
     return hipGetLastError();
 }
 
