@@ -98,8 +98,10 @@ struct CaseMix {
     int npost;                    // dt-refresh blocks of this case (fixed 1024-cell chunks)
     int seg_rows;                 // rows per row segment of the stage kernel's work items
     double *u64[2];               // U^n / U^{n+1}, 5 planes each
-    float *sb;                    // FP32 base dU^n = U^n - U_inf, 5 planes
-    float *s32[2];                // FP32 stage states dU^(k), 5 planes each (ping-pong)
+    // FP32 states, 5 components each: components 0-3 of cell c at [4c .. 4c+3], component 4
+    // at [4 plane + c] (a row loads with one 16-byte and one 4-byte access per lane)
+    float *sb;                    // FP32 base dU^n = U^n - U_inf
+    float *s32[2];                // FP32 stage states dU^(k) (ping-pong)
     // FP32 geometry, interleaved so that the row march loads it with two wide loads per row:
     // tau2[2f + 0/1] = tau-normal (nx, ny) of face f, (ni+1) x pitch faces; cell4[4c + 0..3] =
     // n-normal (nx, ny), V and 1/d^2 of cell c, ni x pitch cells (1/V is formed from V)

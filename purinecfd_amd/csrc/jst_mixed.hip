@@ -62,7 +62,7 @@ __device__ __noinline__ void far_state64(const CaseMix &C, const float *S, int j
         const int c = cidx(ni - 1 - m, jw, pitch);
         double u[5];
 #pragma unroll
-        for (int q = 0; q < 5; ++q) u[q] = C.k.u64[q] + (double)S[q * plane + c];
+        for (int q = 0; q < 5; ++q) u[q] = C.k.u64[q] + (double)(q < 4 ? S[4 * c + q] : S[4 * plane + c]);
         const double inv = 1.0 / u[0];
         vx[m] = u[1] * inv;
         vy[m] = u[2] * inv;
@@ -362,6 +362,25 @@ __device__ __forceinline__ void bst(__amdgpu_buffer_rsrc_t r, unsigned q, unsign
 {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, q * 8u, soff, 0);
 }
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+// An FP32 state array (dU, 5 components) of plane cells: components 0-3 of cell q as a quad
+// at byte 16 q, component 4 in a plane after the quads, at byte 16 plane + 4 q.  A row's
+// state is then one 16-byte and one 4-byte load (store) per lane instead of five.
+__device__ __forceinline__ void sld5(__amdgpu_buffer_rsrc_t r, unsigned q, unsigned pb16, float v[5])
+{
+    const f32x4 a = bld4(r, q, 0u);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = a.z;
+    v[3] = a.w;
+    v[4] = bld(r, q, pb16);
+}
+__device__ __forceinline__ void sst5(__amdgpu_buffer_rsrc_t r, unsigned q, unsigned pb16, const float v[5])
+{
+    const f32x4 a = {v[0], v[1], v[2], v[3]};
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, a), r, q * 16u, 0u, 0);
+    bst(r, q, pb16, v[4]);
+}
 
 } // namespace
 
@@ -470,8 +489,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             if (ST || r < ni) {
                 const int rr = ST ? (r < ni ? r : ni - 1) : (r >= 0 ? r : -1 - r); // -1-r: wall mirror
                 const unsigned q = cidx(rr, jw, pitch);
-#pragma unroll
-                for (int m = 0; m < 5; ++m) bu[m] = bld(r_src, q, m * pb4);
+                sld5(r_src, q, 4u * pb4, bu);
             }
             // only what the segment's faces and cells use: V/dt of rows is-1 .. ie, tau
             // normals of faces is-1 .. ie+1, cell geometry of rows is-1 .. ie
@@ -496,8 +514,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
             if (MODE != MIX_TERMS && (ST || (c >= is && c < ie))) {
                 const unsigned q = cidx(c, jw, pitch);
                 if (MODE == MIX_INNER) {
-#pragma unroll
-                    for (int m = 0; m < 5; ++m) bsb[m] = bld(r_sb, q, m * pb4);
+                    sld5(r_sb, q, 4u * pb4, bsb);
                 }
                 if (MODE == MIX_LAST) {
 #pragma unroll
@@ -678,8 +695,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     const float rho = k.rho_inf + nw[0], mx = k.m_inf + nw[1], my = k.n_inf + nw[2];
                     const float prho = k.gm1 * ((k.e_inf + nw[3]) * rho - 0.5f * (mx * mx + my * my));
                     if (own) {
-#pragma unroll
-                        for (int m = 0; m < 5; ++m) bst(r_dst, q, m * pb4, nw[m]);
+                        sst5(r_dst, q, 4u * pb4, nw);
                         // NonPhysical (solver.rs:151-185, state.rs:241-259), also catching NaN
                         if (!(rho > 1e-15f) || !(prho > 1e-15f * rho))
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
@@ -714,11 +730,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     racc += own ? dr * dr : 0.0;
                     dacc = own ? fminf(dacc, ldt) : dacc;
                     if (own) {
+                        float nb[5];
 #pragma unroll
                         for (int m = 0; m < 5; ++m) {
                             bst(r_un, q, m * pb8, u1[m]);
-                            bst(r_sb, q, m * pb4, (float)(u1[m] - k.u64[m]));
+                            nb[m] = (float)(u1[m] - k.u64[m]);
                         }
+                        sst5(r_sb, q, 4u * pb4, nb);
                         bst(r_g, q, o_ldt_next, ldt);
                         bst(r_g, q, o_rt_next, (conv + acou) * k.inv_cfl);
                         if (bad)
@@ -950,10 +968,12 @@ __global__ void shadow_kernel(const CaseMix *__restrict__ cases, int ci)
     const CaseMix &C = cases[ci];
     const int rot = C.ctl->rot;
     const size_t plane = (size_t)C.ni * C.pitch;
+    // the FP32 state layout: components 0-3 as quads per cell, component 4 as a plane after
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < 5 * plane;
          q += (size_t)gridDim.x * blockDim.x) {
         const int m = (int)(q / plane);
-        C.sb[q] = (float)(C.u64[rot][q] - C.k.u64[m]);
+        const size_t c = q - (size_t)m * plane;
+        C.sb[m < 4 ? 4 * c + m : 4 * plane + c] = (float)(C.u64[rot][q] - C.k.u64[m]);
     }
 }
 
