@@ -18,7 +18,7 @@ and grid of the stage kernel, picked by the autotuner.  Reported:
 * roofline: the dominant kernel, the FP32 row-march stage of RK stages 1-4 (4 of the 5
   launches per step): 152 algorithmic B/cell-stage x cells / its mean launch duration (HIP
   events around every launch on the batch's stream); stage 5 and the step end beside, and
-  the bytes this design's launches move (92 B/cell inner, 160 stage 5);
+  the bytes this design's launches move (88 B/cell inner, 156 stage 5);
 * cpu_baseline: the oracle (C restatement of jst-rs, OpenMP) timed on this host's cores on a
   bounded sample (one 257x129 case of the batch).
 
@@ -45,11 +45,12 @@ HBM_PEAK_GBS = 8000.0
 #                    + FP32 geometry 28 + V/dt 4
 # (a leaner storage scheme shows up as a higher rate, not as a smaller figure).  What this
 # build's launches actually have to move, per cell, is reported beside it:
-#   mixed RK stages 1-4: dU^(k-1) 20 + dU^n 20 + dU^(k) 20 + FP32 geometry 28 + V/dt 4 = 92
-#   mixed stage 5: dU^(4) 20 + U^n 40 + U^{n+1} 40 + dU^{n+1} 20 + geometry 28 + V/dt 4
-#                  + next dt and V/dt 8 = 160
+#   mixed RK stages 1-4: dU^(k-1) 20 + dU^n 20 + dU^(k) 20 + FP32 geometry 24 (tau and n
+#                        normals, V, 1/d^2; 1/V is formed from V) + V/dt 4 = 88
+#   mixed stage 5: dU^(4) 20 + U^n 40 + U^{n+1} 40 + dU^{n+1} 20 + geometry 24 + V/dt 4
+#                  + next dt and V/dt 8 = 156
 BYTES = {"fp64": 184.0, "mixed": 152.0}
-DESIGN_BYTES = {"mixed_inner": 92.0, "mixed_last": 160.0}
+DESIGN_BYTES = {"mixed_inner": 88.0, "mixed_last": 156.0}
 METRIC = "cell-stage updates/s FP32-batched vs FP64-plain (ΔCL,ΔCD≤1e-4); p99 solve latency"
 
 
