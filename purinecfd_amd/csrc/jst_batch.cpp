@@ -108,7 +108,8 @@ CaseLayout layout_for(int ni, int nj, int pitch, int npost, int res_cap)
     L.cell4 = carve(plane * 16);
     for (int b = 0; b < 2; ++b) L.ratio[b] = carve(plane * 4);
     for (int b = 0; b < 2; ++b) L.localdt[b] = carve(plane * 4);
-    const size_t nparts = (size_t)ni * ((nj + MIX_COLS - 1) / MIX_COLS); // 1-row blocks at most
+    // one partial per (row segment, column group): at most ni segments (1-row segments)
+    const size_t nparts = (size_t)ni * ((nj + MIX_COLS - 1) / MIX_COLS);
     L.rpart = carve(nparts * 8);
     L.dpart = carve(nparts * 8);
     L.res_hist = carve((size_t)res_cap * 8);
