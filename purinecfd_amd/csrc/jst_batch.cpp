@@ -671,7 +671,7 @@ jst_status jst_batch_add(jst_batch *b, const char *config_json, const char *mesh
         return fail(JST_ERR_ARG, "the mixed path needs at least 3 cell rings and 4 cells per ring");
     // 32-bit byte offsets: every array of a case (at most 6 planes of (NI+1) x pitch, 8
     // bytes), a 5-plane FP64 state (40 B per cell) and the FP32 geometry / V/dt / local dt
-    // span the stage kernels address through one descriptor (44 B per cell) stay below 4 GiB
+    // span the stage kernels address through one descriptor (40 B per cell) stay below 4 GiB
     if ((size_t)mesh.n_rings * ((mesh.n_theta + 31) / 32 * 32) * 6 * 8 >= ((size_t)1 << 32))
         return fail(JST_ERR_ARG, "mesh too large for 32-bit array offsets");
     int slot = -1;

@@ -23,11 +23,6 @@
 
 namespace {
 
-__device__ __forceinline__ int wrapj(int j, int nj)
-{
-    j %= nj;
-    return j < 0 ? j + nj : j;
-}
 // 32-bit cell offsets: the host rejects cases whose planes exceed 2^31 / 8 elements
 __device__ __forceinline__ unsigned cidx(int i, int j, int pitch) { return (unsigned)(i * pitch + j); }
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
