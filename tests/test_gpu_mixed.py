@@ -192,6 +192,10 @@ def test_batch_fitted_segment_length(jst):
     for c in ids[1:]:
         b.remove(c)
     assert b.segment_rows(ids[0]) == 1  # refitted to the one case left
+    with pytest.raises(jst.JstError):
+        b.segment_rows(ids[1])  # removed
+    with pytest.raises(jst.JstError):
+        b.segment_rows(999)
     b.close()
 
     cases = workloads.c4_batch()
