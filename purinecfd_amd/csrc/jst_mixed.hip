@@ -320,11 +320,11 @@ template <class T> __device__ __forceinline__ void stg(T *p, unsigned q, T v)
     *(gptr<T>)((gptr<char>)p + q * (unsigned)sizeof(T)) = v;
 }
 
-// A 5-plane array (component m lies m planes past the base) through one buffer descriptor:
-// every component takes the same 32-bit lane offset and a scalar plane offset (soffset).
-// Five 64-bit bases per array held more scalar registers than the row loop has, and the
-// loop restored the spilled ones with a lane read and a wait at every access.  The
-// descriptor spans the array (5 planes), raw addressing (stride 0), the gfx9 word 3.
+// A multi-plane array through one buffer descriptor: every plane takes the same 32-bit lane
+// offset and a scalar plane offset (soffset).  Five 64-bit bases per array held more scalar
+// registers than the row loop has, and the loop restored the spilled ones with a lane read
+// and a wait at every access.  The descriptor spans the array, raw addressing (stride 0),
+// the gfx9 word 3.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t planes_rsrc(const void *p, unsigned bytes)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, 0x00020000);
@@ -449,7 +449,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         while (jw >= nj) jw -= nj;
         const bool own = lane >= 3 && lane < 61 && jg < nj;
 
-        // the 5-plane arrays (wave-uniform descriptors) and the byte offsets of their planes
+        // the 5-component state arrays (wave-uniform descriptors; the FP32 ones hold quads then a
+        // plane of component 4, at byte 4 pb4; the FP64 ones five planes pb8 bytes apart)
         const unsigned pb4 = plane * 4u, pb8 = plane * 8u;
         const __amdgpu_buffer_rsrc_t r_src = planes_rsrc(src, 5u * pb4), r_sb = planes_rsrc(C.sb, 5u * pb4),
                                      r_dst = planes_rsrc(sdst, 5u * pb4),
