@@ -477,6 +477,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         float bu[5] = {}, brt = 0.f, btn[2] = {}, bnn[2] = {}, bvol = 0.f, biw = 0.f;
         float bsb[5] = {}, bldt = 0.f;
         double bfm[5] = {}; // stage 5: U^n of the row the step updates
+        // stages 1-4: the last updated row's stores, issued at the top of the next row step
+        float pv[5] = {};
+        unsigned pq = 0;
+        bool pend = false;
 
         // Loads of row r's bundle.  In a steady step every condition below holds except that
         // row r may be the first far-field ghost row: its (unused) state load is clamped.
@@ -537,6 +541,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                 un[m] = bu[m];
                 base[m] = bsb[m];
                 former[m] = bfm[m];
+            }
+            if (MODE == MIX_INNER) {
+                // Wait for everything outstanding (the bundle loaded one row step ago, the
+                // stores issued at the top of that step), then issue the previous row's stores
+                // and this step's prefetch.  Without this wait, the wait the compiler places
+                // for the previous bundle counts the stores too (loads and stores share one
+                // counter) and lands after the new prefetch, waiting on it as well.
+                __builtin_amdgcn_s_waitcnt(0x0F70); // vmcnt(0); expcnt, lgkmcnt untouched
+                if (pend) {
+                    sst5(r_dst, pq, 4u * pb4, pv);
+                    pend = false;
+                }
             }
             if (ST || r < ie + 2) load_bundle(r + 1, steady);
             // keep the prefetch and U^n loads ahead of the row's arithmetic (the scheduler
@@ -691,7 +707,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     const float rho = k.rho_inf + nw[0], mx = k.m_inf + nw[1], my = k.n_inf + nw[2];
                     const float prho = k.gm1 * ((k.e_inf + nw[3]) * rho - 0.5f * (mx * mx + my * my));
                     if (own) {
-                        sst5(r_dst, q, 4u * pb4, nw);
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) pv[m] = nw[m];
+                        pq = q;
+                        pend = true;
                         // NonPhysical (solver.rs:151-185, state.rs:241-259), also catching NaN
                         if (!(rho > 1e-15f) || !(prho > 1e-15f * rho))
                             atomicMin(&ctl->err_key, ((unsigned long long)seq << 32) |
@@ -804,18 +823,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         int r = is - 3;
         load_bundle(r, edge_t{});
         for (; r < rs0 && r <= ie + 2; ++r) row_step(r, edge_t{});
-        // stage 5: two steady rows per trip, so that the row windows rename across the pair
-        // instead of moving (820 instead of 889 instructions per row; 1.6 % faster).  Stages
-        // 1-4 also drop from 670 to 607 instructions per row that way, but run 10 % slower
-        // (105 -> 115 us on the 64-case batch, with or without a scheduling barrier between
-        // the two rows).
-        if (MODE == MIX_LAST)
+        // Two steady rows per trip, so that the row windows rename across the pair instead of
+        // moving (stage 5: 820 instead of 889 instructions per row, 1.6 % faster; stages 1-4:
+        // 609 instead of 666, and with the explicit wait and the stores one row step late,
+        // 83.5 instead of 86.4 us per launch on the 64-case batch; pairs without them were
+        // 10 % slower than single rows).
+        if (MODE == MIX_LAST || MODE == MIX_INNER)
             for (; r + 1 <= rs1; r += 2) {
                 row_step(r, steady_t{});
                 row_step(r + 1, steady_t{});
             }
         for (; r <= rs1; ++r) row_step(r, steady_t{});
         for (; r <= ie + 2; ++r) row_step(r, edge_t{});
+        if (MODE == MIX_INNER && pend) sst5(r_dst, pq, 4u * pb4, pv);
     }
 }
 
