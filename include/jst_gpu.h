@@ -283,7 +283,8 @@ typedef struct {
     double admit_s, done_s;   /* seconds after the loop starts */
     long steps;
     int status;               /* jst_case_status at retirement, or -1 if never admitted */
-    double residual, cl, cd;
+    double residual, cl, cd;  /* cl, cd: NaN for a request that ends non-physical or is
+                                 never admitted */
 } jst_request_result;
 
 typedef struct {

@@ -33,6 +33,7 @@ void jo_config_defaults_ext(jo_config *c)
     c->local_time_stepping = 0;
     c->residual_drop = 0.0;
     c->viscous_time_step = 0;
+    c->sa_negative = 0;
 }
 
 /* config.rs:153-156: mu0 * (r*sqrt r) * (T0+Ts) / (T+Ts) */
@@ -599,7 +600,7 @@ double jo_timestep(jo_solver *s)
                 double rnu = rho * s->nut[k];
                 double chi = rnu / mu, c3 = chi * chi * chi;
                 double mut = rnu * (c3 / (c3 + s->d.cv1_cubed));
-                if (!(mut > 0.0)) mut = 0.0;
+                if (!(mut > 0.0) || (s->cfg.sa_negative && rnu < 0.0)) mut = 0.0;
                 double g = s->cfg.gamma > 4.0 / 3.0 ? s->cfg.gamma : 4.0 / 3.0;
                 double lv = g / rho * (mu * s->d.inv_pr + mut * s->d.inv_prt) *
                             (a * a + b * b + cc * cc + dd * dd) / s->vol[k];
@@ -727,6 +728,15 @@ static void diffusion_tensor(const jo_solver *s, double t, double rho_nu, double
     double c3 = chi * chi * chi;
     double f1 = c3 / (c3 + d->cv1_cubed);
     double mu_t = rho_nu * f1;
+    double nu_coef = rho_nu;
+    if (s->cfg.sa_negative && rho_nu < 0.0) {
+        /* EXT (ICCFD7-1902 eqs. 20-21): no eddy viscosity, and the nu~ diffusion
+         * coefficient mu + rho nu~ fn with fn = (cn1 + chi^3) / (cn1 - chi^3), cn1 = 16.
+         * It applies to the wall ghosts too (nu~ = -nu~ of the first cell), where the
+         * reference's fv1 has its pole at chi = +cv1 of the first cell. */
+        mu_t = 0.0;
+        nu_coef = rho_nu * ((16.0 + c3) / (16.0 - c3));
+    }
     double mu_eff = mu + mu_t;
     double lam_eff = mu * d->inv_pr + mu_t * d->inv_prt;
     double txx = mu_eff * (4.0 / 3.0 * g[0] - 2.0 / 3.0 * g[3]);
@@ -734,7 +744,7 @@ static void diffusion_tensor(const jo_solver *s, double t, double rho_nu, double
     double txy = mu_eff * (g[1] + g[2]);
     double qx = lam_eff * d->cp * g[4];
     double qy = lam_eff * d->cp * g[5];
-    double nu_diff = s->cfg.sigma * (mu + rho_nu);
+    double nu_diff = s->cfg.sigma * (mu + nu_coef);
     aux[0] = mu;
     aux[1] = chi;
     aux[2] = f1;
@@ -954,6 +964,12 @@ void jo_source(jo_solver *s)
                             cb1_over_kappa2 * ft2) *
                            rho * (nu_tilde * nu_tilde * inv_dist2);
             double diff = cb2_sigma * rho * (g[6] * g[6] + g[7] * g[7]);
+            if (sa->sa_negative && rho_nu < 0.0) {
+                /* EXT (ICCFD7-1902 eq. 19): production cb1 (1 - ct3) S rho nu~ on the
+                 * vorticity S, destruction -cw1 rho (nu~/d)^2 */
+                prod = sa->cb1 * (1.0 - sa->ct3) * sv * rho_nu;
+                destr = -d->cw1 * rho * (nu_tilde * nu_tilde * inv_dist2);
+            }
             s->src[k] = (prod - destr + diff) * s->vol[k];
         }
     }

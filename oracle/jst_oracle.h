@@ -19,7 +19,7 @@
  *   - invariants (metric closure, free-stream preservation, mass conservation,
  *     thread-count independence).
  *   - the extensions marked EXT below (nut_ratio, reynolds, local time stepping,
- *     residual_drop, forces) are not present in the reference: parity unpinned beyond the
+ *     residual_drop, viscous_time_step, sa_negative, forces) are not present in the reference: parity unpinned beyond the
  *     closed-form checks listed in the tests (default values reproduce the reference).
  */
 #ifndef JST_ORACLE_H
@@ -66,6 +66,9 @@ typedef struct {
     int local_time_stepping; /* 0: global min dt (solver.rs:110,115) */
     double residual_drop;    /* decades below step-1 residual; <= 0: off */
     int viscous_time_step;   /* 0: convective spectral radius only (timestep.rs:24-47) */
+    int sa_negative;         /* 0: fv1, source and nu~ diffusion as viscous.rs / source.rs for
+                              * every sign of nu~; 1: the negative-nu~ branch of Allmaras,
+                              * Johnson & Spalart (ICCFD7-1902, 2012) where nu~ < 0 */
 } jo_config;
 
 /* Derived constants (config.rs:114-146, finish 173-206). */

@@ -4,7 +4,8 @@ The oracle restates jst-rs (src/*.rs) in plain C; see ``oracle/jst_oracle.h``.  
 parsing here follows jst-rs config.rs:100-206 (serde field names, optional ``solver``
 section with defaults 10000 / 1e-10); the default-off extension keys are read from the same
 JSON (``simulation.Re``, ``simulation.ref_length``, ``simulation.local_time_stepping``,
-``simulation.viscous_time_step``, ``spalart_allmaras.nut_ratio``, ``solver.residual_drop``).
+``simulation.viscous_time_step``, ``spalart_allmaras.nut_ratio``,
+``spalart_allmaras.negative``, ``solver.residual_drop``).
 """
 from __future__ import annotations
 
@@ -34,7 +35,7 @@ class JoConfig(C.Structure):
         ("k2", C.c_double), ("k4", C.c_double),
         ("nut_ratio", C.c_double), ("reynolds", C.c_double), ("ref_length", C.c_double),
         ("local_time_stepping", C.c_int), ("residual_drop", C.c_double),
-        ("viscous_time_step", C.c_int),
+        ("viscous_time_step", C.c_int), ("sa_negative", C.c_int),
     ]
 
 
@@ -133,6 +134,7 @@ def config_from_dict(doc: dict) -> JoConfig:
     c.local_time_stepping = 1 if sim.get("local_time_stepping", False) else 0
     c.residual_drop = float(sol.get("residual_drop", 0.0))
     c.viscous_time_step = 1 if sim.get("viscous_time_step", False) else 0
+    c.sa_negative = 1 if sa.get("negative", False) else 0
     return c
 
 

@@ -185,6 +185,7 @@ MixConst make_mix_const(const Config &c, const Derived &d)
     m.rmax = (float)c.rmax;
     m.lts = c.lts ? 1 : 0;
     m.visc_dt = c.visc_dt ? 1 : 0;
+    m.sa_neg = c.sa_neg ? 1 : 0;
     return m;
 }
 
@@ -204,6 +205,7 @@ struct jst_batch {
     int ntiles = 0;
     int *d_slots = nullptr; // slots of the batch's cases (one step-end block each)
     int nslots = 0;
+    bool neg = false;       // some case of the lists runs the negative-nu~ SA branch
     int variant = 0, grid = 0;
     int fit_rows = 24; // segment length under MIX_VARIANT_FIT, refitted by rebuild_lists
     bool dirty = true;
@@ -323,12 +325,14 @@ jst_status rebuild_lists(jst_batch *b)
 {
     std::vector<TileDesc> nat;
     std::vector<int> slots;
+    bool neg = false;
     if (b->variant == MIX_VARIANT_FIT) b->fit_rows = fit_segment_rows(b);
     for (auto &hc : b->cases) {
         if (!hc) continue;
         const int rows = case_rows(b, *hc);
         case_items(*hc, rows, nat);
         slots.push_back(hc->slot);
+        neg = neg || hc->cfg.sa_neg;
         // stage 5 leaves one step-end partial per (segment, column group): the segment length
         // is read by stage 5 and the step end
         if (hc->dev.seg_rows != rows) {
@@ -359,6 +363,7 @@ jst_status rebuild_lists(jst_batch *b)
     HIPCHK(hipStreamSynchronize(b->stream));
     b->ntiles = (int)tiles.size();
     b->nslots = (int)slots.size();
+    b->neg = neg;
     jst_status st = drop_graph(b);
     if (st != JST_OK) return st;
     b->dirty = false;
@@ -371,7 +376,7 @@ jst_status launch_one_step(jst_batch *b)
 {
     for (int k = 0; k < 5; ++k)
         HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k], false,
-                                       b->stream));
+                                       b->neg, b->stream));
     HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
     return JST_OK;
 }
@@ -903,7 +908,8 @@ jst_status jst_batch_residual_terms(jst_batch *b, int case_id, double *dt_out)
     HIPCHK(hipMalloc(&dt, t.size() * sizeof(TileDesc)));
     std::unique_ptr<TileDesc, void (*)(TileDesc *)> guard(dt, [](TileDesc *p) { (void)hipFree(p); });
     HIPCHK(hipMemcpyAsync(dt, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice, b->stream));
-    HIPCHK(jst::launch_stage_mixed(b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true, b->stream));
+    HIPCHK(jst::launch_stage_mixed(b->d_cases, dt, (int)t.size(), 0, 0, RK_ALPHA[0], true,
+                                   hc->cfg.sa_neg, b->stream));
     jst_status st = fetch_ctl(b); // synchronises the stream before the list is freed
     if (st != JST_OK) return st;
     if (dt_out) *dt_out = b->h_ctl[case_id].dt;
@@ -932,7 +938,8 @@ jst_status jst_batch_residual_terms_noalloc(jst_batch *b, int case_id, double *d
     if (cur != b->device) return fail(JST_ERR_ARG, "the batch's device is not the current device");
     int n = 0;
     HIPCHK(jst::launch_make_items(b->noalloc_items, hc->slot, hc->ni, hc->nj, rows, &n, b->stream));
-    HIPCHK(jst::launch_stage_mixed(b->d_cases, b->noalloc_items, n, 0, 0, RK_ALPHA[0], true, b->stream));
+    HIPCHK(jst::launch_stage_mixed(b->d_cases, b->noalloc_items, n, 0, 0, RK_ALPHA[0], true,
+                                   hc->cfg.sa_neg, b->stream));
     jst_status st = fetch_ctl(b); // waits for the stream
     if (st != JST_OK) return st;
     if (dt_out) *dt_out = b->h_ctl[case_id].dt;
@@ -1093,7 +1100,7 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
             for (int k = 0; k < 5; ++k) {
                 HIPCHK(hipEventRecord(evs[10 * s + 2 * k], b->stream));
                 HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
-                                               false, b->stream));
+                                               false, b->neg, b->stream));
                 HIPCHK(hipEventRecord(evs[10 * s + 2 * k + 1], b->stream));
             }
             HIPCHK(jst::launch_end_mixed(b->d_cases, b->d_slots, b->nslots, b->stream));
@@ -1128,7 +1135,7 @@ jst_status jst_batch_time_kernels(jst_batch *b, int nsteps, double *inner_ms, do
         for (int k = 0; k < 5; ++k) {
             HIPCHK(hipEventRecord(evs[12 * s + 2 * k], b->stream));
             HIPCHK(jst::launch_stage_mixed(b->d_cases, b->d_tiles, b->ntiles, b->grid, k, RK_ALPHA[k],
-                                           false, b->stream));
+                                           false, b->neg, b->stream));
             HIPCHK(hipEventRecord(evs[12 * s + 2 * k + 1], b->stream));
         }
         HIPCHK(hipEventRecord(evs[12 * s + 10], b->stream));

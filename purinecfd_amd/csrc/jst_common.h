@@ -26,6 +26,7 @@ struct Config {
     double nut_ratio = 0.1, reynolds = 0.0, ref_length = 1.0, residual_drop = 0.0;
     bool lts = false;
     bool visc_dt = false; // simulation.viscous_time_step: viscous spectral radius in the local dt
+    bool sa_neg = false;  // spalart_allmaras.negative: negative-nu~ branch of the SA model
 };
 
 struct Derived {

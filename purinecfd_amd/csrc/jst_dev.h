@@ -41,6 +41,7 @@ struct PhysConst {
     double targetres, residual_drop;
     int supersonic, lts;
     int visc_dt; // the local dt also holds the viscous spectral radius (default off)
+    int sa_neg;  // negative-nu~ branch of the SA model where nu~ < 0 (default off)
 };
 
 // Device view of one case.  FP64 plain path: every array is double.
@@ -87,7 +88,7 @@ struct MixConst {
     float rho_inf, u_inf, v_inf, p_inf, t_inf, m_inf, n_inf, e_inf, h_inf, rnu_inf, nut_inf;
     float suth_num, inv_t_ref, t_suth, inv_pr, inv_prt, cp, sigma, cv1_cubed;
     float cb1, cb2, ct3, ct4, cw1, cw2, cw3_6, one_plus_cw3_6, fv3, inv_kappa2, rmax;
-    int lts, visc_dt;
+    int lts, visc_dt, sa_neg;
     // FP64 copies for the accumulation and the far-field state
     double u64[5];                // U_inf (conserved)
     PhysConst k;                  // full FP64 constants (far field, step control)

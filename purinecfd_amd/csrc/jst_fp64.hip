@@ -180,7 +180,7 @@ __device__ __forceinline__ double visc_radius(const PhysConst &k, double rho, do
     const double mu = r * sqrt(r) * k.suth_num / (t + k.t_suth); // config.rs:210-213
     const double chi = rho_nu / mu, c3 = chi * chi * chi;
     double mut = rho_nu * (c3 / (c3 + k.cv1_cubed));
-    if (!(mut > 0.0)) mut = 0.0;
+    if (!(mut > 0.0) || (k.sa_neg && rho_nu < 0.0)) mut = 0.0;
     const double g = k.gamma > 4.0 / 3.0 ? k.gamma : 4.0 / 3.0;
     return g / rho * (mu * k.inv_pr + mut * k.inv_prt) * (a * a + b * b + c * c + d * d) / vol;
 }
@@ -197,6 +197,15 @@ __device__ __forceinline__ void diffusion_tensor(const PhysConst &k, double t, d
     double c3 = chi * chi * chi;
     double f1 = c3 / (c3 + k.cv1_cubed);
     double mu_t = rho_nu * f1;
+    double nu_coef = rho_nu;
+    if (k.sa_neg && rho_nu < 0.0) {
+        // extension spalart_allmaras.negative (Allmaras, Johnson & Spalart, ICCFD7-1902,
+        // eqs. 20-21): no eddy viscosity, nu~ diffusion mu + rho nu~ fn with
+        // fn = (cn1 + chi^3) / (cn1 - chi^3), cn1 = 16; also for the wall ghosts (nu~ =
+        // -nu~ of the first cell), where the reference's fv1 has its pole at chi = +cv1
+        mu_t = 0.0;
+        nu_coef = rho_nu * ((16.0 + c3) / (16.0 - c3));
+    }
     double mu_eff = mu + mu_t;
     double lam_eff = mu * k.inv_pr + mu_t * k.inv_prt;
     double txx = mu_eff * (4.0 / 3.0 * g[0] - 2.0 / 3.0 * g[3]);
@@ -204,7 +213,7 @@ __device__ __forceinline__ void diffusion_tensor(const PhysConst &k, double t, d
     double txy = mu_eff * (g[1] + g[2]);
     double qx = lam_eff * k.cp * g[4];
     double qy = lam_eff * k.cp * g[5];
-    double nu_diff = k.sigma * (mu + rho_nu);
+    double nu_diff = k.sigma * (mu + nu_coef);
     d[0] = txx;
     d[1] = txy;
     d[2] = vx * txx + vy * txy + qx;
@@ -244,6 +253,12 @@ __device__ __forceinline__ double sa_source(const PhysConst &k, double rho, doub
     double fw = gg * cbrt(sqrt(k.one_plus_cw3_6 / (pow6(gg) + k.cw3_6)));
     double destr = (k.cw1 * fw - cb1_over_kappa2 * ft2) * rho * (nu_tilde * nu_tilde * inv_dist2);
     double diff = cb2_sigma * rho * (g[6] * g[6] + g[7] * g[7]);
+    if (k.sa_neg && rho_nu < 0.0) {
+        // extension spalart_allmaras.negative (ICCFD7-1902 eq. 19): production on the
+        // vorticity cb1 (1 - ct3) S rho nu~, destruction -cw1 rho (nu~ / d)^2
+        prod = k.cb1 * (1.0 - k.ct3) * s * rho_nu;
+        destr = -k.cw1 * rho * (nu_tilde * nu_tilde * inv_dist2);
+    }
     return (prod - destr + diff) * vol;
 }
 

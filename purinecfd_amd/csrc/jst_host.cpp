@@ -345,6 +345,11 @@ jst_status parse_config(const char *text, Config &c)
     num(sa, "kappa", c.kappa);
     num(sa, "rmax", c.rmax);
     optnum(sa, "nut_ratio", c.nut_ratio);
+    if (sa) {
+        const JVal *v = sa->get("negative");
+        if (v && v->kind == JVal::BOOL) c.sa_neg = v->b;
+        else if (v && err.empty()) err = "invalid type for `negative`: expected bool";
+    }
     const JVal *dis = sec("dissipation");
     num(dis, "k2", c.k2);
     num(dis, "k4", c.k4);
@@ -629,6 +634,7 @@ PhysConst make_const(const Config &c, const Derived &d)
     k.supersonic = d.supersonic ? 1 : 0;
     k.lts = c.lts ? 1 : 0;
     k.visc_dt = c.visc_dt ? 1 : 0;
+    k.sa_neg = c.sa_neg ? 1 : 0;
     return k;
 }
 

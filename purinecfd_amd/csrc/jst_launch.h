@@ -43,8 +43,10 @@ hipError_t launch_forces_fp64(const CaseDev *cases, int ci, double *scratch, dou
 #define MIX_MAX_ROWS 256
 #define MIX_COLS 58
 extern const int MIX_SEGMENT_ROWS[MIX_NVARIANTS];
+// neg: launch the kernels compiled with the negative-nu~ SA branch (needed when any case of
+// the item list has spalart_allmaras.negative; bitwise the same results for the others)
 hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
-                              int stage, double alpha, bool terms, hipStream_t s);
+                              int stage, double alpha, bool terms, bool neg, hipStream_t s);
 hipError_t launch_end_mixed(const CaseMix *cases, const int *slots, int nslots, hipStream_t s);
 hipError_t launch_dt_mixed(const CaseMix *cases, const PostDesc *posts, int nposts, hipStream_t s);
 hipError_t launch_to_f32(const double *a, float *b, size_t n, hipStream_t s);

@@ -154,6 +154,9 @@ __device__ __forceinline__ void euler_flux_pert(const MixConst &k, const float a
 }
 
 // jst-rs viscous.rs:37-71 in FP32; g = gradients of (du, dv, dT, nu~) == of (u, v, T, nu~).
+// NEG: compiled with the negative-nu~ branch (taken where the case has it switched on; the
+// wall ghosts included).
+template <bool NEG>
 __device__ __forceinline__ void diffusion_tensor32(const MixConst &k, float dT, float drn, float du,
                                                    float dv, const float g[8], float d[8],
                                                    float &chi_out, float &fv1_out)
@@ -166,13 +169,20 @@ __device__ __forceinline__ void diffusion_tensor32(const MixConst &k, float dT, 
     const float chi = rnu * frcp(mu);
     const float c3 = chi * chi * chi;
     const float f1 = c3 * frcp(c3 + k.cv1_cubed);
-    const float mu_t = rnu * f1;
+    float mu_t = rnu * f1;
+    float nu_coef = rnu;
+    if (NEG && k.sa_neg) { // wave-uniform: extension spalart_allmaras.negative (ICCFD7-1902 eqs. 20-21)
+        if (rnu < 0.0f) {
+            mu_t = 0.0f;
+            nu_coef = rnu * (16.0f + c3) * frcp(16.0f - c3);
+        }
+    }
     const float mu_eff = mu + mu_t;
     const float lam = mu * k.inv_pr + mu_t * k.inv_prt;
     const float txx = mu_eff * (4.0f / 3.0f * g[0] - 2.0f / 3.0f * g[3]);
     const float tyy = mu_eff * (4.0f / 3.0f * g[3] - 2.0f / 3.0f * g[0]);
     const float txy = mu_eff * (g[1] + g[2]);
-    const float nu_diff = k.sigma * (mu + rnu);
+    const float nu_diff = k.sigma * (mu + nu_coef);
     d[0] = txx;
     d[1] = txy;
     d[2] = u * txx + v * txy + lam * k.cp * g[4];
@@ -194,6 +204,7 @@ __device__ __noinline__ double fw64(const PhysConst &k, double r)
 }
 
 // jst-rs source.rs:45-88 in FP32 (x^(1/6) via exp2/log2).
+template <bool NEG>
 __device__ __forceinline__ float sa_source32(const MixConst &k, float drho, float drn, float nu,
                                              float chi, float fv1, const float g[8], float inv_d2,
                                              float vol)
@@ -205,10 +216,10 @@ __device__ __forceinline__ float sa_source32(const MixConst &k, float drho, floa
     const float fv2 = 1.0f - chi * frcp(1.0f + chi * fv1);
     const float s = k.fv3 * fabsf(g[2] - g[1]);
     const float st = fmaxf(s + nu * inv_kd2 * fv2, 1e-10f);
-    const float prod = k.cb1 * (1.0f - ft2) * st * rnu;
+    float prod = k.cb1 * (1.0f - ft2) * st * rnu;
     const float r = fminf(nu * frcp(st) * inv_kd2, k.rmax);
     float fw;
-    if (r >= -8.0f) {
+    if (r >= -8.0f || (NEG && k.sa_neg)) { // with the negative branch, r < 0 lanes do not use fw
         // g^6 stays below 1e32 for r in [-8, rmax = 10]: FP32 is safe
         const float r2 = r * r, r6 = r2 * r2 * r2;
         const float gg = r + k.cw2 * (r6 - r);
@@ -220,14 +231,21 @@ __device__ __forceinline__ float sa_source32(const MixConst &k, float drho, floa
         // FP32 (inf * 0), so this rare case takes the FP64 formula of the plain path
         fw = (float)fw64(k.k, (double)r);
     }
-    const float destr = (k.cw1 * fw - k.cb1 * k.inv_kappa2 * ft2) * rho * (nu * nu * inv_d2);
+    float destr = (k.cw1 * fw - k.cb1 * k.inv_kappa2 * ft2) * rho * (nu * nu * inv_d2);
     const float diff = k.cb2 * k.sigma * rho * (g[6] * g[6] + g[7] * g[7]);
+    if (NEG && k.sa_neg) { // wave-uniform: extension spalart_allmaras.negative (ICCFD7-1902 eq. 19)
+        if (rnu < 0.0f) {
+            prod = k.cb1 * (1.0f - k.ct3) * s * rnu;
+            destr = -k.cw1 * rho * (nu * nu * inv_d2);
+        }
+    }
     return (prod - destr + diff) * vol;
 }
 
 // Viscous spectral radius of a cell (extension simulation.viscous_time_step; Blazek,
 // Computational Fluid Dynamics, 3rd ed., eqs. 6.21-6.22): max(4/3, gamma)/rho (mu/Pr +
 // mu_t/Prt) (|A|^2 + |C|^2)/V, mu_t >= 0.  FP32 from the new state: rho, 1/rho, p, rho nu~.
+template <bool NEG>
 __device__ __forceinline__ float visc_radius32(const MixConst &k, float rho, float inv, float p, float rnu,
                                                float a, float b, float c, float d, float vol)
 {
@@ -235,7 +253,7 @@ __device__ __forceinline__ float visc_radius32(const MixConst &k, float rho, flo
     const float r = t * k.inv_t_ref;
     const float mu = r * fsqrt(r) * k.suth_num * frcp(t + k.t_suth);
     const float chi = rnu * frcp(mu), c3 = chi * chi * chi;
-    const float mut = fmaxf(rnu * c3 * frcp(c3 + k.cv1_cubed), 0.0f);
+    const float mut = (NEG && k.sa_neg && rnu < 0.0f) ? 0.0f : fmaxf(rnu * c3 * frcp(c3 + k.cv1_cubed), 0.0f);
     const float g = k.gamma > 4.0f / 3.0f ? k.gamma : 4.0f / 3.0f;
     return g * inv * (mu * k.inv_pr + mut * k.inv_prt) * (a * a + b * b + c * c + d * d) * frcp(vol);
 }
@@ -248,7 +266,7 @@ __device__ __forceinline__ double visc_radius64(const PhysConst &k, double rho, 
     const double mu = r * sqrt(r) * k.suth_num / (t + k.t_suth);
     const double chi = rho_nu / mu, c3 = chi * chi * chi;
     double mut = rho_nu * (c3 / (c3 + k.cv1_cubed));
-    if (!(mut > 0.0)) mut = 0.0;
+    if (!(mut > 0.0) || (k.sa_neg && rho_nu < 0.0)) mut = 0.0;
     const double g = k.gamma > 4.0 / 3.0 ? k.gamma : 4.0 / 3.0;
     return g / rho * (mu * k.inv_pr + mut * k.inv_prt) * (a * a + b * b + c * c + d * d) / vol;
 }
@@ -397,7 +415,7 @@ __device__ __forceinline__ void sst5(__amdgpu_buffer_rsrc_t r, unsigned q, unsig
 // on the case's segment length but never on the launch grid or on the other cases.
 // MIX_TERMS: export R = Fc - Fv - Fd and S of the current state (no update).
 // ---------------------------------------------------------------------------------------
-template <int MODE>
+template <int MODE, bool NEG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mix_march_kernel(const CaseMix *__restrict__ cases,
                                                         const TileDesc *__restrict__ items, int nitems,
                                                         int stage, double alpha)
@@ -647,15 +665,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     }
                 }
                 float chi, fv1;
-                diffusion_tensor32(k, PR[1][2], U[3][4], PR[1][0], PR[1][1], g, D[2], chi, fv1);
+                diffusion_tensor32<NEG>(k, PR[1][2], U[3][4], PR[1][0], PR[1][1], g, D[2], chi, fv1);
                 if (ST || (x >= is && x < ie && x < ni))
-                    S[2] = sa_source32(k, U[3][0], U[3][4], PR[1][3], chi, fv1, g, iw_new, VOL[2]);
+                    S[2] = sa_source32<NEG>(k, U[3][0], U[3][4], PR[1][3], chi, fv1, g, iw_new, VOL[2]);
                 if (!ST && x == 0) {
                     // ghost row -1 takes the gradient of row 0 without its temperature part
                     g[4] = 0.0f;
                     g[5] = 0.0f;
                     float chi_g, fv1_g;
-                    diffusion_tensor32(k, PR[0][2], U[2][4], PR[0][0], PR[0][1], g, D[1], chi_g, fv1_g);
+                    diffusion_tensor32<NEG>(k, PR[0][2], U[2][4], PR[0][0], PR[0][1], g, D[1], chi_g, fv1_g);
                 }
             }
 
@@ -738,7 +756,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
                     const float acou = cs * (fsqrt(aa * aa + bb * bb) + fsqrt(cc * cc + dd * dd));
                     float lam = conv + acou;
                     if (k.visc_dt) // wave-uniform: the viscous spectral radius joins the local dt
-                        lam += 4.0f * visc_radius32(k, rho, inv, pr, (float)u1[4], aa, bb, cc, dd, VOL[0]);
+                        lam += 4.0f * visc_radius32<NEG>(k, rho, inv, pr, (float)u1[4], aa, bb, cc, dd, VOL[0]);
                     const float ldt = k.cfl * VOL[0] / lam;
                     const double dr = u1[0] - former[0];
                     // this column's residual sum and dt minimum over the segment
@@ -1058,21 +1076,31 @@ hipError_t launch_make_items(TileDesc *items, int slot, int ni, int nj, int rows
 
 const int MIX_SEGMENT_ROWS[MIX_NVARIANTS] = {16, 24, 32, 40, 48, 64, 8, 4, 2, 1};
 
+template <bool NEG>
+static void launch_stage_mixed_t(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
+                                 int stage, double alpha, bool terms, hipStream_t s)
+{
+    if (terms)
+        hipLaunchKernelGGL((mix_march_kernel<MIX_TERMS, NEG>), dim3(grid), dim3(256), 0, s, cases, items,
+                           nitems, stage, alpha);
+    else if (stage == 4)
+        hipLaunchKernelGGL((mix_march_kernel<MIX_LAST, NEG>), dim3(grid), dim3(256), 0, s, cases, items,
+                           nitems, stage, alpha);
+    else
+        hipLaunchKernelGGL((mix_march_kernel<MIX_INNER, NEG>), dim3(grid), dim3(256), 0, s, cases, items,
+                           nitems, stage, alpha);
+}
+
 hipError_t launch_stage_mixed(const CaseMix *cases, const TileDesc *items, int nitems, int grid,
-                              int stage, double alpha, bool terms, hipStream_t s)
+                              int stage, double alpha, bool terms, bool neg, hipStream_t s)
 {
     const int need = (nitems + 3) / 4; // four waves (work items) per workgroup
     if (grid <= 0 || grid > need) grid = need;
     if (grid <= 0) return hipSuccess;
-    if (terms)
-        hipLaunchKernelGGL((mix_march_kernel<MIX_TERMS>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
-                           stage, alpha);
-    else if (stage == 4)
-        hipLaunchKernelGGL((mix_march_kernel<MIX_LAST>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
-                           stage, alpha);
+    if (neg)
+        launch_stage_mixed_t<true>(cases, items, nitems, grid, stage, alpha, terms, s);
     else
-        hipLaunchKernelGGL((mix_march_kernel<MIX_INNER>), dim3(grid), dim3(256), 0, s, cases, items, nitems,
-                           stage, alpha);
+        launch_stage_mixed_t<false>(cases, items, nitems, grid, stage, alpha, terms, s);
     return hipGetLastError();
 }
 

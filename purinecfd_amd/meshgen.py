@@ -8,12 +8,15 @@ point of every ring at its end, which Mesh::parse trims (mesh.rs:169-189).
 * ``ellipse_mesh`` is the family of jst-py/tools/genmesh.py:26-73 (ellipse wall -> circular
   far field, geometric radial stretch); a_wall == b_wall gives a circular cylinder.
 * ``naca_mesh`` is an algebraic O-mesh around a NACA 4-digit section with a closed trailing
-  edge: cosine clustering along the chord, straight radial lines to a far-field circle
-  centred at mid-chord, geometric radial spacing set by the first wall spacing.
+  edge: cosine clustering along the chord, grid lines that leave the wall along its normal
+  and bend over to a far-field circle centred at mid-chord, geometric radial spacing set by
+  the first wall spacing.
 """
 from __future__ import annotations
 
 import math
+
+import numpy as np
 
 
 def _svals_geometric(n_rings: int, stretch: float) -> list[float]:
@@ -95,20 +98,40 @@ def naca4(m: float, p: float, t: float, x: float) -> tuple[float, float, float, 
             x + yt * math.sin(th), yc - yt * math.cos(th))
 
 
+def _smooth_directions(v: np.ndarray, sigma: float) -> np.ndarray:
+    """Unit vectors v (n x 2, periodic in the first index) convolved with a periodic Gaussian
+    of standard deviation ``sigma`` points, renormalised."""
+    n = v.shape[0]
+    k = np.fft.fftfreq(n)
+    g = np.exp(-2.0 * (math.pi * sigma * k) ** 2)
+    out = np.fft.ifft(np.fft.fft(v, axis=0) * g[:, None], axis=0).real
+    return out / np.linalg.norm(out, axis=1, keepdims=True)
+
+
 def naca_mesh(n_theta: int, n_rings: int, m: float = 0.0, p: float = 0.4, t: float = 0.12,
-              first_spacing: float = 1e-5, r_far: float = 30.0,
-              closed: bool = False) -> tuple[str, float]:
+              first_spacing: float = 1e-5, r_far: float = 30.0, closed: bool = False,
+              wall_sigma: float = 3.0, far_sigma: float = 0.125) -> tuple[str, float]:
     """Algebraic O-mesh around a NACA 4-digit section (chord 1, LE at x=0, TE at x=1).
 
     Wall points j = 0..n_theta-1 follow theta_j = 2 pi j / n_theta: x = (1 + cos theta)/2,
     upper surface for theta < pi (counter-clockwise from the trailing edge), lower surface
-    after.  Ring i is wall + s_i (far - wall) with the far point at mid-chord + r_far
-    (cos theta, sin theta) and s_i geometric with s_1 * r_far == first_spacing.
-    Returns (mesh text, stretch ratio)."""
+    after.  N_j is the outward wall normal from the central difference of the wall ring (at
+    the trailing-edge point the bisector), smoothed over ``wall_sigma`` points so that the
+    grid lines fan out around the sharp trailing edge; F_j is where the ray from W_j along
+    N_j smoothed over ``far_sigma`` of the ring meets the far-field circle (centre (0.5, 0),
+    radius r_far; the wide smoothing keeps these rays from crossing under a concave lower
+    surface).  Ring i, with s_i geometric and s_1 r_far = first_spacing, is
+
+        P_ij = W_j + (1 - s_i) s_i r_far N_j + s_i^2 (F_j - W_j),
+
+    i.e. normal to the wall at the wall (first spacing first_spacing (1 - s_1) + O(s_1^2)),
+    the straight line W_j -> F_j at the far field.  Grid lines that meet the wall at a
+    shallow angle (straight lines to far-field points spaced evenly in angle meet it at 10
+    degrees next to the trailing edge) make the explicit scheme unstable at the trailing-edge
+    wall cells.  Returns (mesh text, stretch ratio)."""
     q = stretch_for_first_spacing(n_rings, r_far, first_spacing)
-    svals = _svals_geometric(n_rings, q)
-    per = n_theta + 1 if closed else n_theta
-    wall, far = [], []
+    svals = np.asarray(_svals_geometric(n_rings, q))
+    wall = []
     for j in range(n_theta):
         th = 2.0 * math.pi * j / n_theta
         x = 0.5 * (1.0 + math.cos(th))
@@ -119,10 +142,26 @@ def naca_mesh(n_theta: int, n_rings: int, m: float = 0.0, p: float = 0.4, t: flo
             wall.append((xu, yu))
         else:
             wall.append((xl, yl))
-        far.append((0.5 + r_far * math.cos(th), r_far * math.sin(th)))
-    pts = []
-    for s in svals:
-        for j in range(per):
-            (wx, wy), (fx, fy) = wall[j % n_theta], far[j % n_theta]
-            pts.append((wx + s * (fx - wx), wy + s * (fy - wy)))
-    return _fmt(pts, n_rings, per, "{:.17g} {:.17g}\n"), q
+    w = np.array(wall)
+    tang = np.roll(w, -1, axis=0) - np.roll(w, 1, axis=0)
+    nrm = np.stack([tang[:, 1], -tang[:, 0]], axis=1)  # outward for a counter-clockwise ring
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    # the smoothed normals blend into the exact ones away from the trailing edge, so that the
+    # curved leading edge keeps grid lines normal to the wall
+    d_te = np.minimum(np.arange(n_theta), n_theta - np.arange(n_theta))
+    wgt = np.exp(-0.5 * (d_te / (2.0 * wall_sigma)) ** 2)[:, None]
+    n_wall = wgt * _smooth_directions(nrm, wall_sigma) + (1.0 - wgt) * nrm
+    n_wall /= np.linalg.norm(n_wall, axis=1, keepdims=True)
+    n_far = _smooth_directions(nrm, far_sigma * n_theta)
+    rel = w - np.array([0.5, 0.0])
+    b = np.sum(rel * n_far, axis=1)
+    dist = -b + np.sqrt(b * b - (np.sum(rel * rel, axis=1) - r_far * r_far))
+    far = w + dist[:, None] * n_far
+    pts = (w[None] + ((1.0 - svals) * svals * r_far)[:, None, None] * n_wall[None]
+           + (svals * svals)[:, None, None] * (far - w)[None])
+    pts[0] = w
+    pts[-1] = far
+    if closed:
+        pts = np.concatenate([pts, pts[:, :1]], axis=1)
+    per = pts.shape[1]
+    return _fmt(pts.reshape(-1, 2).tolist(), n_rings, per, "{:.17g} {:.17g}\n"), q

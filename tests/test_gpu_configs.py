@@ -17,6 +17,7 @@ field-normed) and tests/properties.rs:49-75 (bitwise independence of the partiti
 from __future__ import annotations
 
 import json
+import math
 
 import numpy as np
 import pytest
@@ -145,19 +146,21 @@ def test_uneven_batch_of_64_cases(jst, oracle_lib):
 
 
 def test_serving_stream_matches_alone_runs_and_the_oracle(jst, oracle_lib):
-    """A seeded 2 s Poisson stream of the serving mix (cylinders 40x17 .. 160x65 to a
-    6-decade drop, draws of the uneven NACA distribution capped at 500 steps), served with
-    each case's own segment length (segment_rows -1): every request ends bitwise where the same
-    case run alone ends (steps, status, residual, CL, CD); one converged cylinder of each size
-    is within 1e-4 of the oracle in CL and CD."""
+    """A seeded 2 s Poisson stream of the serving mix (cylinders 40x17 .. 160x65 with the
+    reference's time stepping, draws of the uneven NACA distribution with local time stepping
+    and the negative-nu~ SA branch, all to a 6-decade drop), served with each case's own
+    segment length (segment_rows -1): every request ends bitwise where the same case run alone
+    ends (steps, status, residual, CL, CD); one converged cylinder of each size and the first
+    converged 129x65 NACA draw are within 1e-4 of the oracle in CL and CD."""
     from purinecfd_amd import workloads
     reqs = workloads.c5_requests(seconds=2.0, rate=20.0, seed=20261204)
     assert len(reqs) >= 20
     res, summ = jst.serve([(r["arrival_s"], r["config"], r["mesh"], r["max_steps"]) for r in reqs],
                           hbm_bytes=16 << 30, max_cases=96, steps_per_round=50, segment_rows=-1)
-    assert summ["completed"] == len(reqs) and summ["rejected"] == 0 and summ["nonphysical"] == 0
+    assert summ["completed"] == len(reqs) and summ["rejected"] == 0
     assert summ["converged"] >= 1 and summ["p99_converged_s"] >= summ["p50_converged_s"] > 0
     checked = set()
+    naca_converged = 0
     for r, out in zip(reqs, res):
         b = jst.Batch(1 << 30, 1)
         b.set_tiling(10, 0)  # each case its own segment length, as in the stream
@@ -167,10 +170,17 @@ def test_serving_stream_matches_alone_runs_and_the_oracle(jst, oracle_lib):
         b.close()
         assert out["steps"] == info["step"] and out["status"] == info["status"], r["name"]
         assert out["residual"] == info["residual"], r["name"]
+        if out["status"] == jst.CASE_NONPHYSICAL:  # the stream reports no forces for those
+            assert math.isnan(out["cl"]) and math.isnan(out["cd"]), r["name"]
+            continue
         assert out["cl"] == f["cl"] and out["cd"] == f["cd"], r["name"]
-        if r["name"].startswith("cyl") and out["status"] == jst.CASE_CONVERGED \
-                and r["name"] not in checked:
-            checked.add(r["name"])
+        if out["status"] != jst.CASE_CONVERGED:
+            continue
+        oracle_check = r["name"] not in checked and (
+            r["name"].startswith("cyl") or ("_129x65" in r["name"] and "naca" not in checked))
+        naca_converged += not r["name"].startswith("cyl")
+        if oracle_check:
+            checked.add(r["name"] if r["name"].startswith("cyl") else "naca")
             o = oracle_lib.OracleSolver(json.loads(r["config"]), r["mesh"])
             orep = o.run(r["max_steps"])
             # the residual history flattens towards a 6-decade drop and the mixed one carries
@@ -180,4 +190,5 @@ def test_serving_stream_matches_alone_runs_and_the_oracle(jst, oracle_lib):
                 (r["name"], orep["steps"], out["steps"])
             of = o.forces()
             assert abs(out["cl"] - of["cl"]) <= 1e-4 and abs(out["cd"] - of["cd"]) <= 1e-4, r["name"]
-    assert checked, "no cylinder converged"
+    assert any(c.startswith("cyl") for c in checked), "no cylinder converged"
+    assert "naca" in checked and naca_converged >= 5, (checked, naca_converged)

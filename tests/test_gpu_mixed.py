@@ -348,25 +348,44 @@ def test_serving_with_grid_tuning_matches_alone(jst):
         b.close()
 
 
-def test_naca_fine_wall_spacing_tracks_fp64(jst):
-    """NACA 0012 257x129 with a 1e-5 c first spacing (M 0.75, Re 6e6): the trailing-edge
-    wall cells turn nu~ negative in the first steps, which drives the SA r to large negative
-    values (the FP32 fw would overflow; the mixed path takes the FP64 formula there).  The
-    mixed path must track the FP64 plain path: 40 steps, |dCL|, |dCD| <= 1e-5, field L2 <= 1e-5."""
+def test_negative_nut_under_the_reference_sa_tracks_fp64(jst):
+    """C4 case 8 (129x65) with local time stepping at CFL 2 and the reference's SA model: nu~
+    turns negative next to the wall from step 195, and from step 241 a few wall cells have a
+    clamped S~ and r near -2e10, where the FP32 fw would overflow (the mixed path takes the
+    FP64 formula there).  On the FP64 path's step-250 state one residual evaluation of the
+    mixed path is within 2e-4 field-normed of the FP64 one (R and S); over the 250 steps the
+    mixed run keeps |dCL|, |dCD| <= 1e-4 and the L2 norms of density, pressure and velocity
+    within 2e-4 (bursts at a wall cell near the leading edge move local values by more)."""
     from purinecfd_amd import workloads
-    w = workloads.c2_naca0012()
-    s64 = jst.Solver(w["config"], w["mesh"], jst.FP64)
-    smx = jst.Solver(w["config"], w["mesh"], jst.MIXED)
-    assert s64.run(40).steps == 40
-    assert smx.run(40).steps == 40
+    w = workloads.c4_batch(n=9)[8]
+    cfg = json.loads(w["config"])
+    cfg["simulation"].update({"CFL": 2.0, "local_time_stepping": True})
+    s64 = jst.Solver(json.dumps(cfg), w["mesh"], jst.FP64)
+    smx = jst.Solver(json.dumps(cfg), w["mesh"], jst.MIXED)
+    assert s64.run(250).steps == 250
+    assert smx.run(250).steps == 250
+    assert s64.get("nut").min() < 0.0
     f, g = smx.forces(), s64.forces()
-    assert abs(f["cl"] - g["cl"]) <= 1e-5 and abs(f["cd"] - g["cd"]) <= 1e-5, (f, g)
-    for n in ("rho", "p", "u", "v"):
-        assert rel_l2(smx.get(n), s64.get(n)) <= 1e-5, n
-    wm, w64 = smx.wall(), s64.wall()
-    assert np.array_equal(wm["x"], w64["x"]) and np.array_equal(wm["y"], w64["y"])
-    assert np.max(np.abs(wm["cp"] - w64["cp"])) <= 1e-4
-    assert np.max(np.abs(wm["cf"] - w64["cf"])) <= 1e-4 * max(1.0, np.max(np.abs(w64["cf"])))
+    errs = {"cl": abs(f["cl"] - g["cl"]), "cd": abs(f["cd"] - g["cd"])}
+    for n in ("rho", "p"):
+        errs[n] = rel_l2(smx.get(n), s64.get(n))
+    dv = np.hypot(smx.get("u") - s64.get("u"), smx.get("v") - s64.get("v"))
+    errs["velocity"] = float(np.linalg.norm(dv) / np.linalg.norm(np.hypot(s64.get("u"), s64.get("v"))))
+    u250 = np.stack([s64.get(f"U{k}") for k in range(5)])
+    smx.set_state(u250)
+    smx.residual_terms()
+    s64.residual_terms()
+    for k in range(5):
+        want = s64.get(f"Fc{k}") - s64.get(f"Fv{k}") - s64.get(f"Fd{k}")
+        errs[f"R{k}"] = float(np.max(np.abs(smx.get(f"R{k}") - want)) / np.max(np.abs(want)))
+    want = s64.get("S")
+    errs["S"] = float(np.max(np.abs(smx.get("S") - want)) / np.max(np.abs(want)))
+    print("mixed vs fp64, reference SA through negative nu~:", {k: f"{v:.1e}" for k, v in errs.items()})
+    assert errs["cl"] <= 1e-4 and errs["cd"] <= 1e-4, errs
+    for n in ("rho", "p", "velocity"):
+        assert errs[n] <= 2e-4, (n, errs[n])
+    for n in ("R0", "R1", "R2", "R3", "R4", "S"):
+        assert errs[n] <= 2e-4, (n, errs[n])
 
 
 def test_serving_under_load_switches_tiling_within_tolerance(jst):

@@ -101,6 +101,54 @@ def test_naca_wall_ring(oracle_lib):
     assert abs(area - exact) < 1e-3 * exact  # polygon of 256 cosine-spaced points
     # symmetric section: the lower surface mirrors the upper one
     assert np.max(np.abs(y[1:] + y[1:][::-1])) < 1e-15
-    # first wall spacing at the leading edge: s_1 |far - wall| = 1e-4 * 29.5 / 30
+    # first wall spacing at the leading edge, where the wall normal and the far-field ray
+    # are both -x and |far - wall| = 29.5: (1 - s_1) s_1 30 + s_1^2 29.5 with s_1 30 = 1e-4
+    s1 = 1e-4 / 30.0
     d = np.hypot(xy[1, 128, 0] - x[128], xy[1, 128, 1] - y[128])
-    assert abs(d - 1e-4 * 29.5 / 30.0) < 1e-10
+    assert abs(d - ((1.0 - s1) * s1 * 30.0 + s1 * s1 * 29.5)) < 1e-15
+
+
+def _cell_areas(xy):
+    """Signed quad areas of an O-mesh node array (rings, n_theta, 2); positive for the
+    orientation the reference assumes (counter-clockwise rings, i outward)."""
+    a, b = xy[:-1], np.roll(xy[:-1], -1, axis=1)
+    c, d = np.roll(xy[1:], -1, axis=1), xy[1:]
+
+    def cross(u, v):
+        return u[..., 0] * v[..., 1] - u[..., 1] * v[..., 0]
+    return -0.5 * (cross(a, b) + cross(b, c) + cross(c, d) + cross(d, a))
+
+
+def _wall_angle(xy):
+    """Degrees by which the first grid line leaves the wall away from its normal (the wall
+    tangent is the central difference of the wall ring)."""
+    t = np.roll(xy[0], -1, axis=0) - np.roll(xy[0], 1, axis=0)
+    r = xy[1] - xy[0]
+    cos = np.abs(np.sum(t * r, axis=1)) / (np.linalg.norm(t, axis=1) * np.linalg.norm(r, axis=1))
+    return np.degrees(np.arcsin(np.clip(cos, 0.0, 1.0)))
+
+
+@pytest.mark.parametrize("n_theta,rings,m,t,h", [(256, 129, 0.0, 0.12, 1e-5),
+                                                  (128, 65, 0.04, 0.08, 1e-3),
+                                                  (512, 257, 0.04, 0.18, 1e-3)])
+def test_naca_grid_lines_leave_the_wall_along_its_normal(oracle_lib, n_theta, rings, m, t, h):
+    """Away from the trailing edge the first grid line is the wall normal (to 0.01 degree);
+    next to the sharp trailing edge the lines fan out (at most 40 degrees off the normal, where
+    straight lines to far-field points evenly spaced in angle leave at 80)."""
+    _, _, xy = _nodes(oracle_lib, meshgen.naca_mesh(n_theta, rings, m, 0.4, t, h, 30.0)[0])
+    ang = _wall_angle(xy)
+    far_from_te = np.abs(((np.arange(n_theta) + n_theta // 2) % n_theta) - n_theta // 2) > 16
+    assert ang[far_from_te].max() < 0.01 and ang.max() < 40.0
+    assert np.all(_cell_areas(xy) > 0.0)
+
+
+def test_every_c4_mesh_has_positive_cells():
+    """The whole 64-case distribution (cambered sections included, where normals from the
+    concave lower surface converge): no folded cell."""
+    from purinecfd_amd import workloads
+    for w in workloads.c4_batch():
+        lines = w["mesh"].split("\n")
+        rings, per = map(int, lines[0].split())
+        xy = np.array([list(map(float, ln.split())) for ln in lines[1:1 + rings * per]])
+        xy = xy.reshape(rings, per, 2)[:, :-1]  # closed rings: drop the closing duplicate
+        assert np.all(_cell_areas(xy) > 0.0), w["name"]
