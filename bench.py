@@ -13,8 +13,12 @@ and grid of the stage kernel, picked by the autotuner.  Reported:
 * accuracy: per case, mixed vs FP64 plain after the same number of steps from the same start:
   max |dCL|, |dCD| and field L2 (the FP64 plain path is pinned to the CPU oracle at 1e-10 per
   residual evaluation by the test suite);
-* serving: p50/p99 time to completion of a Poisson request stream (mean 20 solves/s) run by
-  the serving loop (jst_serve);
+* converged_accuracy: the same 64 geometries and flows with the settings that converge them
+  (local time stepping at CFL 2, the negative-nu~ SA branch), run to a 6-decade drop in one
+  mixed batch and on the FP64 plain path one case at a time: outcome per case, |dCL|, |dCD|
+  and field L2 where both converge, and the time to solution of the whole distribution;
+* serving: p50/p99 time to completion, and to convergence over the converged requests, of a
+  Poisson request stream (mean 20 solves/s) run by the serving loop (jst_serve);
 * roofline: the dominant kernel, the FP32 row-march stage of RK stages 1-4 (4 of the 5
   launches per step): 152 algorithmic B/cell-stage x cells / its mean launch duration (HIP
   events around every launch on the batch's stream); stage 5 and the step end beside, and
@@ -158,6 +162,59 @@ def mixed_accuracy_pass(jst, cases, acc_steps: int):
     return res
 
 
+def converged_accuracy_leg(jst, cases, max_steps: int = 20000):
+    """The C4 distribution with the settings that converge it (local time stepping at CFL 2,
+    the negative-nu~ SA branch, 6-decade drop, at most max_steps): all 64 cases in one mixed
+    batch, and the FP64 plain path one case at a time, each to its own convergence; per case
+    the outcome on both paths and, where both converge, |dCL|, |dCD| and the field L2."""
+    import numpy as np
+    from purinecfd_amd import workloads
+    cfgs = [workloads.converging_config(w, max_steps) for w in cases]
+    b = jst.Batch(hbm_bytes=16 << 30, max_cases=len(cases))
+    ids = [b.add(c, w["mesh"], max_steps) for c, w in zip(cfgs, cases)]
+    t0 = time.perf_counter()
+    b.run(steps_per_round=500)
+    t_mixed = time.perf_counter() - t0
+    names = {jst.CASE_CONVERGED: "converged", jst.CASE_STOPPED: "step cap",
+             jst.CASE_NONPHYSICAL: "non-physical"}
+    rows, both, t_plain, n_mixed, n_plain = [], [], 0.0, 0, 0
+    for w, c, cid in zip(cases, cfgs, ids):
+        im = b.info(cid)
+        fm = b.forces(cid) if im["status"] != jst.CASE_NONPHYSICAL else None
+        fields = {n: b.get(cid, n, w["ni"], w["nj"]) for n in ("rho", "u", "v", "p")}
+        s = jst.Solver(c, w["mesh"], jst.FP64)
+        t0 = time.perf_counter()
+        try:
+            rep = s.run(max_steps)
+            st, steps = (jst.CASE_CONVERGED if rep.converged else jst.CASE_STOPPED), rep.steps
+        except jst.NonPhysical:
+            st, steps = jst.CASE_NONPHYSICAL, s.step
+        t_plain += time.perf_counter() - t0
+        row = [w["name"], names[im["status"]], im["step"], names[st], steps]
+        n_mixed += im["status"] == jst.CASE_CONVERGED
+        n_plain += st == jst.CASE_CONVERGED
+        if im["status"] == jst.CASE_CONVERGED and st == jst.CASE_CONVERGED:
+            f = s.forces()
+            l2 = max(float(np.linalg.norm(fields[n] - s.get(n)) / np.linalg.norm(s.get(n)))
+                     for n in fields)
+            both.append((abs(fm["cl"] - f["cl"]), abs(fm["cd"] - f["cd"]), l2))
+            row += both[-1]
+        rows.append(row)
+        s.close()
+    b.close()
+    return {"settings": "local time stepping, CFL 2, negative-nu~ SA branch, 6-decade drop, "
+                        f"at most {max_steps} steps",
+            "converged_mixed": n_mixed, "converged_fp64": n_plain, "converged_both": len(both),
+            "max_abs_dcl": max(r[0] for r in both) if both else None,
+            "max_abs_dcd": max(r[1] for r in both) if both else None,
+            "max_field_l2_rel": max(r[2] for r in both) if both else None,
+            "time_to_solution_s": {"mixed_batch_all_64": t_mixed, "fp64_one_at_a_time": t_plain},
+            "reference": "FP64 plain GPU path (oracle-pinned), each case to its own convergence",
+            "per_case_columns": ["case", "mixed", "mixed_steps", "fp64", "fp64_steps", "abs_dcl",
+                                 "abs_dcd", "max_field_l2_rel"],
+            "per_case": rows}
+
+
 def serving_leg(jst, seconds: float, rate: float, seed: int):
     from purinecfd_amd import workloads
     reqs = workloads.c5_requests(seconds=seconds, rate=rate, seed=seed)
@@ -255,6 +312,8 @@ def bench_c4(args, jst):
     }
     if args.throughput_only:
         return out
+    if not args.no_converged:
+        out["converged_accuracy"] = converged_accuracy_leg(jst, cases)
     if not args.no_serve:
         out["serving"] = serving_leg(jst, args.serve_seconds, 20.0, 20261204)
     if not args.no_cpu:
@@ -309,6 +368,8 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-converged", action="store_true",
+                    help="C4: skip the run of every case to convergence on both paths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--throughput-only", action="store_true",
                     help="C4: only the batched throughput pass (for profiler runs)")

@@ -360,6 +360,7 @@ def test_negative_nut_under_the_reference_sa_tracks_fp64(jst):
     w = workloads.c4_batch(n=9)[8]
     cfg = json.loads(w["config"])
     cfg["simulation"].update({"CFL": 2.0, "local_time_stepping": True})
+    cfg["spalart_allmaras"]["negative"] = False
     s64 = jst.Solver(json.dumps(cfg), w["mesh"], jst.FP64)
     smx = jst.Solver(json.dumps(cfg), w["mesh"], jst.MIXED)
     assert s64.run(250).steps == 250
