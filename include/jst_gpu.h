@@ -63,7 +63,21 @@ typedef void (*jst_step_cb)(long step, double residual, void *user);
  * (mesh.rs:27-44, state.rs:249-257).  Never NULL. */
 const char *jst_last_error(void);
 
-/* Config::from_str (config.rs:165-170) + Config::finish (173-206) on the host only.
+/* config.json follows the crate's schema (config.rs:10-72).  Default-off extension keys,
+ * absent from the reference (left out, they reproduce it exactly):
+ *   simulation.Re, simulation.ref_length   Reynolds number (by a mu0 rescale) and the
+ *                                          reference length of Re and CL/CD (default 1)
+ *   simulation.local_time_stepping (bool)  each cell advances with its own dt
+ *   simulation.viscous_time_step (bool)    the viscous spectral radius joins the local dt
+ *   spalart_allmaras.nut_ratio             nu~ / nu of the free stream (reference: 0.1)
+ *   spalart_allmaras.negative (bool)       the negative-nu~ branch of the SA model
+ *                                          (Allmaras, Johnson & Spalart, ICCFD7-1902):
+ *                                          no eddy viscosity, fn-scaled nu~ diffusion and
+ *                                          a vorticity-based source where nu~ < 0, wall
+ *                                          ghosts included
+ *   solver.residual_drop                   stop once residual < residual(step 1) 10^-drop
+ *
+ * Config::from_str (config.rs:165-170) + Config::finish (173-206) on the host only.
  * derived_out (may be NULL) receives 20 doubles: cv cp cw1 cv1_cubed c_inf u_inf v_inf
  * rho_inf mu_inf nut_inf supersonic suth_num inv_t_ref inv_pr inv_prt inv_kappa2 cw3_6
  * one_plus_cw3_6 mu0_eff halo.  No GPU is touched. */
