@@ -14,7 +14,8 @@ and grid of the stage kernel, picked by the autotuner.  Reported:
   max |dCL|, |dCD| and field L2 (the FP64 plain path is pinned to the CPU oracle at 1e-10 per
   residual evaluation by the test suite);
 * converged_accuracy: the same 64 geometries and flows with the settings that converge them
-  (local time stepping at CFL 2, the negative-nu~ SA branch), run to a 6-decade drop in one
+  (local time stepping at CFL 2 with the viscous spectral radius, the negative-nu~ SA
+  branch), run to a 6-decade drop in one
   mixed batch and on the FP64 plain path one case at a time: outcome per case, |dCL|, |dCD|
   and field L2 where both converge, and the time to solution of the whole distribution;
 * serving: p50/p99 time to completion, and to convergence over the converged requests, of a
@@ -163,8 +164,9 @@ def mixed_accuracy_pass(jst, cases, acc_steps: int):
 
 
 def converged_accuracy_leg(jst, cases, max_steps: int = 20000):
-    """The C4 distribution with the settings that converge it (local time stepping at CFL 2,
-    the negative-nu~ SA branch, 6-decade drop, at most max_steps): all 64 cases in one mixed
+    """The C4 distribution with the settings that converge it (workloads.converging_config:
+    local time stepping at CFL 2 with the viscous spectral radius, the negative-nu~ SA branch,
+    6-decade drop, at most max_steps): all 64 cases in one mixed
     batch, and the FP64 plain path one case at a time, each to its own convergence; per case
     the outcome on both paths and, where both converge, |dCL|, |dCD| and the field L2."""
     import numpy as np
@@ -202,8 +204,8 @@ def converged_accuracy_leg(jst, cases, max_steps: int = 20000):
         rows.append(row)
         s.close()
     b.close()
-    return {"settings": "local time stepping, CFL 2, negative-nu~ SA branch, 6-decade drop, "
-                        f"at most {max_steps} steps",
+    return {"settings": "local time stepping with the viscous spectral radius, CFL 2, "
+                        f"negative-nu~ SA branch, 6-decade drop, at most {max_steps} steps",
             "converged_mixed": n_mixed, "converged_fp64": n_plain, "converged_both": len(both),
             "max_abs_dcl": max(r[0] for r in both) if both else None,
             "max_abs_dcd": max(r[1] for r in both) if both else None,
