@@ -80,6 +80,19 @@ def test_missing_field_is_a_config_error(jst):
         jst.Config.from_str("{not json")
 
 
+def test_extension_switches_are_bools(jst):
+    """The default-off switches are JSON booleans; another type is a config error naming the
+    key, like the crate's serde errors for its own fields."""
+    for sect, key in (("simulation", "local_time_stepping"), ("simulation", "viscous_time_step"),
+                      ("spalart_allmaras", "negative")):
+        doc = config_dict()
+        doc[sect][key] = True
+        jst.Config.from_str(json.dumps(doc))
+        doc[sect][key] = 1
+        with pytest.raises(jst.ConfigError, match=key):
+            jst.Config.from_str(json.dumps(doc))
+
+
 def test_reynolds_extension_sets_the_reynolds_number(jst):
     doc = config_dict()
     doc["simulation"]["Re"] = 1e3

@@ -1,7 +1,8 @@
 """The oracle under AddressSanitizer and UndefinedBehaviorSanitizer (host code only): a short
 run of the golden 17x40 cylinder case through every oracle entry point the other tests use
 (config, mesh parsing and its error paths, steps, residual terms, forces, the wall
-distribution, field export, a state injection that must report NonPhysical), in a child
+distribution, field export, the default-off extensions on a state with negative nu~, a state
+injection that must report NonPhysical), in a child
 process that preloads the compiler's libasan.  Any invalid access, leak-free or not, or
 undefined behaviour aborts the child with a report."""
 from __future__ import annotations
@@ -43,6 +44,19 @@ for name in ("rho", "p", "u", "v", "nut", "Fc0", "Fv1", "Fd2", "S", "vol", "tau_
 f = s.forces()
 w = s.wall()
 assert np.all(np.isfinite(w["cp"])) and np.all(np.isfinite(w["cf"]))
+# the default-off extensions on one short run: local and viscous time steps, the
+# negative-nu~ SA branch on a state whose nu~ is negative in a band of rows
+ext = json.loads(json.dumps(cfg))
+ext["simulation"].update({"local_time_stepping": True, "viscous_time_step": True})
+ext["spalart_allmaras"]["negative"] = True
+x = po.OracleSolver(ext, text)
+ux = np.stack([s.get("U%d" % k) for k in range(5)])
+ux[4, 40:200] = -ux[4, 40:200]
+x.set_state(ux)
+assert x.run(3)["steps"] == 3 and np.all(np.isfinite(x.get("nut")))
+x.timestep()
+x.residual_terms()
+assert np.all(np.isfinite(x.get("S"))) and np.all(np.isfinite(x.get("Fv4")))
 u = np.stack([s.get("U%d" % k) for k in range(5)])
 u[0, 3] = -1.0
 try:
