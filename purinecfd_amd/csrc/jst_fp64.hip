@@ -411,10 +411,15 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
     const int i0 = tl.i0, j0 = tl.j0;
     const int tid = threadIdx.x;
 
-    const int status = *(volatile int *)&ctl->status;
-    const long long step = *(volatile long long *)&ctl->step;
-    const unsigned long long ek = *(volatile unsigned long long *)&ctl->err_key;
-    const int rot = *(volatile int *)&ctl->rot;
+    // control words as they stand when the launch starts (the step end that writes them is an
+    // earlier launch on the stream; within this launch only err_key changes, atomically through
+    // the global pointer, and no tile needs its new value): scalar loads through the constant
+    // address space, issued together instead of as dependent vector loads
+    const auto *cc = (const __attribute__((address_space(4))) CaseCtl *)ctl;
+    const int status = cc->status;
+    const long long step = cc->step;
+    const unsigned long long ek = cc->err_key;
+    const int rot = cc->rot;
     const long long seq = step * 5 + stage;
     if (!TERMS) {
         if (status != JST_RUN_ACTIVE) return;
@@ -440,7 +445,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
             dst = C.ubuf[b1];
         }
     }
-    const double dt = ctl->dt;
+    const double dt = cc->dt;
     // frozen V/dt and local dt of this step: plane (step & 1); stage 5 writes the next step's
     // pair into the other plane while neighbouring tiles still read this one
     const size_t cur = (size_t)(step & 1) * plane, nxt = (size_t)((step + 1) & 1) * plane;

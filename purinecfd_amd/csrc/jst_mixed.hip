@@ -435,12 +435,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         const int ngroups = (nj + MIX_COLS - 1) / MIX_COLS;
         // stage 5: this item's row segment (segments start at multiples of the case's length)
         const int seg = MODE == MIX_LAST ? is / C.seg_rows : 0;
-        // control words: read once per item, made wave-uniform so that every pointer and
-        // branch derived from them lives in scalar registers
-        const int status = __builtin_amdgcn_readfirstlane(*(volatile int *)&ctl->status);
-        const long long step = *(volatile long long *)&ctl->step;
-        const unsigned long long ek = *(volatile unsigned long long *)&ctl->err_key;
-        const int rot = __builtin_amdgcn_readfirstlane(*(volatile int *)&ctl->rot);
+        // control words, read once per item as they stand when the launch starts: the step end
+        // that writes them is an earlier launch on the stream, and within this launch only
+        // err_key changes (atomically, through the global pointer), whose new value no item
+        // needs.  Through the constant address space they are scalar loads issued together
+        // (one round trip instead of five dependent vector loads).
+        const auto *cc = (const __attribute__((address_space(4))) CaseCtl *)ctl;
+        const int status = cc->status;
+        const long long step = cc->step;
+        const unsigned long long ek = cc->err_key;
+        const int rot = cc->rot;
+        const double ctl_dt = cc->dt;
         const long long seq = step * 5 + stage;
         if (MODE != MIX_TERMS) {
             if (status != JST_RUN_ACTIVE) continue; // wave-uniform
@@ -459,7 +464,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void m
         const unsigned o_c4 = goff(C.cell4),
                        o_rt = goff(C.ratio[rot]), o_rt_next = goff(C.ratio[rot ^ 1]),
                        o_ldt = goff(C.localdt[rot]), o_ldt_next = goff(C.localdt[rot ^ 1]);
-        const double a_dt = alpha * ctl->dt;
+        const double a_dt = alpha * ctl_dt;
 
         const int jg = J0 - 3 + lane; // unwrapped column of this lane
         int jw = jg;
