@@ -451,17 +451,48 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
     const size_t cur = (size_t)(step & 1) * plane, nxt = (size_t)((step + 1) & 1) * plane;
 
     // ---- phase 1: R3 load with ghost synthesis ----
-    for (int idx = tid; idx < H3 * W3; idx += NT) {
+    // A thread's interior and wall-mirror cells are loaded first, all together, so that their
+    // round trips to memory overlap instead of following one another's unpack.
+    constexpr int NL3 = (H3 * W3 + NT - 1) / NT;
+    double ul[NL3][5];
+#pragma unroll
+    for (int s = 0; s < NL3; ++s) {
+        const int idx = tid + s * NT;
+        if (idx >= H3 * W3) continue;
+        const int r = idx / W3, c = idx - r * W3;
+        const int i = i0 - JST_H + r;
+        if (i >= -JST_H && i < ni)
+            load_u(src, plane, cidx(i >= 0 ? i : -1 - i, wrapj(j0 - JST_H + c, nj), pitch), ul[s]);
+    }
+    // frozen V/dt on R1: rows clamped to [0, NI-1] (dissipation.rs:59), columns wrapped; loaded
+    // with the state, stored to LDS after it
+    constexpr int NL1 = (H1 * W1 + NT - 1) / NT;
+    double rl[NL1];
+#pragma unroll
+    for (int s = 0; s < NL1; ++s) {
+        const int idx = tid + s * NT;
+        if (idx >= H1 * W1) continue;
+        const int r = idx / W1, c = idx - r * W1;
+        int i = i0 - 1 + r;
+        i = i < 0 ? 0 : (i > ni - 1 ? ni - 1 : i);
+        rl[s] = C.ratio[cur + cidx(i, wrapj(j0 - 1 + c, nj), pitch)];
+    }
+#pragma unroll
+    for (int s = 0; s < NL3; ++s) {
+        const int idx = tid + s * NT;
+        if (idx >= H3 * W3) continue;
         const int r = idx / W3, c = idx - r * W3;
         const int i = i0 - JST_H + r, jj = j0 - JST_H + c;
         const int jw = wrapj(jj, nj);
         double u[5];
         Prim q;
         if (i >= 0 && i < ni) {
-            load_u(src, plane, cidx(i, jw, pitch), u);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) u[m] = ul[s][m];
             unpack(k, u, q);
         } else if (i < 0 && i >= -JST_H) {
-            load_u(src, plane, cidx(-1 - i, jw, pitch), u);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) u[m] = ul[s][m];
             unpack(k, u, q);
             q.vx = -q.vx;
             q.vy = -q.vy;
@@ -482,12 +513,12 @@ __global__ __launch_bounds__(NT) void stage_kernel(const CaseDev *__restrict__ c
         sm.q.vy[r][c] = q.vy;
         sm.q.nu[r][c] = q.nut;
     }
-    // frozen V/dt on R1: rows clamped to [0, NI-1] (dissipation.rs:59), columns wrapped.
-    for (int idx = tid; idx < H1 * W1; idx += NT) {
+#pragma unroll
+    for (int s = 0; s < NL1; ++s) {
+        const int idx = tid + s * NT;
+        if (idx >= H1 * W1) continue;
         const int r = idx / W1, c = idx - r * W1;
-        int i = i0 - 1 + r;
-        i = i < 0 ? 0 : (i > ni - 1 ? ni - 1 : i);
-        sm.ratio[r][c] = C.ratio[cur + cidx(i, wrapj(j0 - 1 + c, nj), pitch)];
+        sm.ratio[r][c] = rl[s];
     }
     __syncthreads();
 
