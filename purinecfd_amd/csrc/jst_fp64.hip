@@ -852,12 +852,17 @@ __global__ __launch_bounds__(NT) void post_end_kernel(const CaseDev *__restrict_
     __shared__ double red_m[NT];
     const CaseDev &C = cases[blockIdx.x];
     const PhysConst &k = C.k;
-    volatile CaseCtl *v = C.ctl;
+    CaseCtl *ctl = C.ctl;
     const int tid = threadIdx.x;
     if (mode == 1) {
-        if (v->status != JST_RUN_ACTIVE) return;
-        if (v->err_key != JST_ERR_NONE) {
-            if (tid == 0) v->status = JST_RUN_NONPHYSICAL;
+        // the stage launches before this one wrote them: scalar loads through the constant
+        // address space, issued together
+        const auto *cc = (const __attribute__((address_space(4))) CaseCtl *)ctl;
+        const int status = cc->status;
+        const unsigned long long ek = cc->err_key;
+        if (status != JST_RUN_ACTIVE) return;
+        if (ek != JST_ERR_NONE) {
+            if (tid == 0) ctl->status = JST_RUN_NONPHYSICAL;
             return;
         }
     }
@@ -878,25 +883,28 @@ __global__ __launch_bounds__(NT) void post_end_kernel(const CaseDev *__restrict_
         __syncthreads();
     }
     if (tid == 0) {
+        // this thread is the only one to touch the control block in this launch: plain
+        // loads (issued together) and stores instead of one round trip per volatile access
+        CaseCtl &v = *ctl;
         if (mode == 1) {
-            const int rot = v->rot;
+            const int rot = v.rot;
             const double res = sqrt(red_s[0] / (double)C.ni / (double)C.nj);
-            v->totaltime = v->totaltime + v->dt;
-            const long long st = v->step + 1;
-            v->step = st;
-            v->rot = rot == 2 ? 0 : rot + 1;
-            v->residual = res;
+            v.totaltime = v.totaltime + v.dt;
+            const long long st = v.step + 1;
+            v.step = st;
+            v.rot = rot == 2 ? 0 : rot + 1;
+            v.residual = res;
             C.res_hist[(st - 1) % C.res_cap] = res;
-            if (st == 1) v->res_first = res;
+            if (st == 1) v.res_first = res;
             bool conv = res < k.targetres;
-            if (k.residual_drop > 0.0 && v->res_first > 0.0)
-                conv = conv || res < v->res_first * pow(10.0, -k.residual_drop);
-            if (!v->timing) {
-                if (conv) v->status = JST_RUN_CONVERGED;
-                else if (st >= v->stop_step) v->status = JST_RUN_STOPPED;
+            if (k.residual_drop > 0.0 && v.res_first > 0.0)
+                conv = conv || res < v.res_first * pow(10.0, -k.residual_drop);
+            if (!v.timing) {
+                if (conv) v.status = JST_RUN_CONVERGED;
+                else if (st >= v.stop_step) v.status = JST_RUN_STOPPED;
             }
         }
-        v->dt = red_m[0];
+        v.dt = red_m[0];
     }
 }
 

@@ -877,7 +877,8 @@ __global__ __launch_bounds__(NT) void mix_end_kernel(const CaseMix *__restrict__
     const CaseMix &C = cases[slots[blockIdx.x]];
     CaseCtl *ctl = C.ctl;
     const int tid = threadIdx.x;
-    if (*(volatile int *)&ctl->status != JST_RUN_ACTIVE) return;
+    // written by earlier launches: a scalar load through the constant address space
+    if (((const __attribute__((address_space(4))) CaseCtl *)ctl)->status != JST_RUN_ACTIVE) return;
     const int n = ((C.ni + C.seg_rows - 1) / C.seg_rows) * ((C.nj + MIX_COLS - 1) / MIX_COLS);
     double s = 0.0, m = __builtin_huge_val();
     // sequential per lane (fixed order); unrolled so that the loads are in flight together
