@@ -18,6 +18,9 @@ and grid of the stage kernel, picked by the autotuner.  Reported:
   branch), run to a 6-decade drop in one
   mixed batch and on the FP64 plain path one case at a time: outcome per case, |dCL|, |dCD|
   and field L2 where both converge, and the time to solution of the whole distribution;
+* c2_convergence: the NACA 0012 257x129 case under its own stop rule (8-decade drop within
+  20 000 steps) on both paths, and on the FP64 path with the step cap raised: the step at
+  which each decade is reached, wall time, CL/CD;
 * serving: p50/p99 time to completion, and to convergence over the converged requests, of a
   Poisson request stream (mean 20 solves/s) run by the serving loop (jst_serve);
 * roofline: the dominant kernel, the FP32 row-march stage of RK stages 1-4 (4 of the 5
@@ -217,6 +220,43 @@ def converged_accuracy_leg(jst, cases, max_steps: int = 20000):
             "per_case": rows}
 
 
+def c2_convergence_leg(jst, long_cap: int = 60000):
+    """The NACA 0012 257x129 case (M 0.75, AoA 2 deg, Re 6e6; local time steps at CFL 2 with the
+    negative-nu~ SA branch) under its own stop rule, an 8-decade drop within 20 000 steps, on
+    both paths, and on the FP64 path again with the step cap raised to long_cap: the step at
+    which each decade is first reached, the wall time of the run and CL/CD."""
+    import json
+    import numpy as np
+    from purinecfd_amd import workloads
+    w = workloads.c2_naca0012()
+    runs = {}
+    for key, prec, cap in (("fp64", jst.FP64, 0), ("mixed", jst.MIXED, 0),
+                           ("fp64_cap_raised", jst.FP64, long_cap)):
+        cfg = w["config"]
+        if cap:
+            c = json.loads(cfg)
+            c["solver"]["iteration"] = cap
+            cfg = json.dumps(c)
+        s = jst.Solver(cfg, w["mesh"], prec)
+        hist = []
+        t0 = time.perf_counter()
+        rep = s.run(on_step=lambda k, r: hist.append(r))
+        wall = time.perf_counter() - t0
+        h = np.array(hist)
+        f = s.forces()
+        runs[key] = {"steps": rep.steps, "converged": rep.converged,
+                     "decades": float(np.log10(h[0] / h.min())),
+                     "first_step_reaching": {str(d): int(np.argmax(h < h[0] * 10.0 ** -d)) + 1
+                                             for d in range(1, 9) if (h < h[0] * 10.0 ** -d).any()},
+                     "wall_s": wall, "cl": f["cl"], "cd": f["cd"]}
+        s.close()
+    return {"case": w["name"], "stop_rule": "8-decade drop of the density residual, at most 20 000 steps",
+            "step_cap_raised_to": long_cap,
+            "abs_dcl_mixed_vs_fp64": abs(runs["mixed"]["cl"] - runs["fp64"]["cl"]),
+            "abs_dcd_mixed_vs_fp64": abs(runs["mixed"]["cd"] - runs["fp64"]["cd"]),
+            **runs}
+
+
 def serving_leg(jst, seconds: float, rate: float, seed: int):
     from purinecfd_amd import workloads
     reqs = workloads.c5_requests(seconds=seconds, rate=rate, seed=seed)
@@ -316,6 +356,7 @@ def bench_c4(args, jst):
         return out
     if not args.no_converged:
         out["converged_accuracy"] = converged_accuracy_leg(jst, cases)
+        out["c2_convergence"] = c2_convergence_leg(jst)
     if not args.no_serve:
         out["serving"] = serving_leg(jst, args.serve_seconds, 20.0, 20261204)
     if not args.no_cpu:
@@ -371,7 +412,7 @@ def main() -> int:
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-converged", action="store_true",
-                    help="C4: skip the run of every case to convergence on both paths")
+                    help="C4: skip the runs to convergence (every C4 case, and the 257x129 case) on both paths")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--throughput-only", action="store_true",
                     help="C4: only the batched throughput pass (for profiler runs)")
