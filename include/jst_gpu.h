@@ -280,6 +280,8 @@ jst_status jst_batch_time_steps(jst_batch *b, int nsteps, double *total_ms, doub
  * RK stage 1-4 launches (inner_ms), the stage-5 launch (last_ms) and the step end (end_ms). */
 jst_status jst_batch_time_kernels(jst_batch *b, int nsteps, double *inner_ms, double *last_ms,
                                   double *end_ms);
+/* Cases of the batch still running (active, no non-physical state) as of the last
+ * synchronisation with the device, and their cells in total. */
 jst_status jst_batch_counts(const jst_batch *b, int *active_cases, long *active_cells);
 /* Profiling calibration (no reference counterpart): one device-to-device copy of `bytes`
  * bytes, so that profiler byte counters can be scaled against a transfer of known size. */
