@@ -19,8 +19,11 @@
  *   - invariants (metric closure, free-stream preservation, mass conservation,
  *     thread-count independence).
  *   - the extensions marked EXT below (nut_ratio, reynolds, local time stepping,
- *     residual_drop, viscous_time_step, sa_negative, forces) are not present in the reference: parity unpinned beyond the
- *     closed-form checks listed in the tests (default values reproduce the reference).
+ *     residual_drop, viscous_time_step, sa_negative, forces) are not present in the
+ *     reference; with their defaults the oracle is the reference's algorithm, and each is
+ *     pinned by closed forms and invariants that a plausible mistake breaks
+ *     (tests/test_oracle_properties.py, tests/test_oracle_sa_negative.py,
+ *     tests/test_oracle_wall.py, tests/test_host_cpu.py).
  */
 #ifndef JST_ORACLE_H
 #define JST_ORACLE_H
